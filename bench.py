@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Separable Gaussian forward + adjoint on one MI355X (Gvoxels/s, % of HBM roofline).
+
+Workload (the largest single-GPU headline configuration): a 256^3 float32
+volume, U[0,1) from a counter-based seed, separable Gaussian
+sigma = (1.5, 1.5, 3.0) -> 11/11/19 taps, reflect border, FP32 accumulation.
+One step = forward (ndl_sepconv_forward) + input adjoint
+(ndl_sepconv_adjoint_input) of a seeded N(0,1) float32 field, inputs resident
+in HBM.  Each launch is timed with HIP events on the stream it runs on; by
+default the Infinity Cache is flushed with a 1 GiB read before every step
+(cold protocol; --warm skips the flush and is reported as warm_value).
+
+roofline: algorithmic bytes per launch = (4 B read + 4 B write) per voxel of
+the fused kernel, divided by its mean launch time, against 8 TB/s.
+cpu_baseline: the reference's own convolve() chain (oracle/_ref, built from
+the reference sources) on one host core, full volume, fwd+adj counted as two
+forward applications (this reflect-Gaussian operator is self-adjoint); falls
+back to the C restatement when the reference build is absent.
+
+    python bench.py [--steps K] [--warmup W] [--n 256] [--warm] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gvoxels/s + % HBM roofline, separable conv fwd/adjoint, 1 MI355X vs host CPU"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--warm", action="store_true", help="no cache flush between steps")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import ndl_amd
+    from ndl_amd import abi, workloads
+
+    n = args.n
+    nvox = n ** 3
+    hip = ndl_amd.hip()
+    stream = hip.stream()
+    lib = ndl_amd.lib()
+    bu = C.CDLL(os.path.join(ROOT, "ndl_amd", "libndl_benchutil.so"))
+    bu.ndl_bench_flush.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+    bu.ndl_bench_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+
+    log(f"bench: building inputs {n}^3 f32")
+    x, axes, taps, borders = workloads.config2a(n=n)
+    w = workloads.normal(2, 7, nvox).astype(np.float32).reshape(x.shape)
+    dx = ndl_amd.DeviceArray.from_numpy(x)
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    dy = ndl_amd.DeviceArray(x.shape, np.float32)
+    dgx = ndl_amd.DeviceArray(x.shape, np.float32)
+    flush_bytes = 1 << 30
+    dflush = ndl_amd.DeviceArray((flush_bytes,), np.uint8)
+    dsink = ndl_amd.DeviceArray((4096,), np.uint32)
+    ndl_amd.check(lib.ndl_memset(dflush.ptr, 1, flush_bytes, stream))
+
+    ks = abi.Kernels(taps, borders)
+    ax = abi.Kernels.axes(axes)
+    tx, ty, tw, tg = dx.tensor(), dy.tensor(), dw.tensor(), dgx.tensor()
+
+    def fwd():
+        ndl_amd.check(lib.ndl_sepconv_forward(C.byref(tx), C.byref(ty), len(axes), ax, ks.arr,
+                                              abi.ACCUM_F32, abi.ROUND_TRUNC, stream))
+
+    def adj():
+        ndl_amd.check(lib.ndl_sepconv_adjoint_input(C.byref(tw), C.byref(tg), len(axes), ax, ks.arr,
+                                                    abi.ACCUM_F32, stream))
+
+    def flush():
+        if bu.ndl_bench_flush(dflush.ptr, flush_bytes, dsink.ptr, stream) != 0:
+            raise RuntimeError("flush launch failed")
+
+    def run(steps, cold):
+        ev = [[hip.event() for _ in range(3)] for _ in range(steps)]
+        ndl_amd.synchronize(stream)
+        t0 = time.perf_counter()
+        for i in range(steps):
+            if cold:
+                flush()
+            hip.record(ev[i][0], stream)
+            fwd()
+            hip.record(ev[i][1], stream)
+            adj()
+            hip.record(ev[i][2], stream)
+        ndl_amd.synchronize(stream)
+        wall = time.perf_counter() - t0
+        f = [hip.elapsed_ms(e[0], e[1]) for e in ev]
+        a = [hip.elapsed_ms(e[1], e[2]) for e in ev]
+        return np.array(f), np.array(a), wall
+
+    log(f"bench: warmup {args.warmup}")
+    run(max(args.warmup, 1), cold=not args.warm)
+    cold = not args.warm
+    log(f"bench: timing {args.steps} steps ({'cold' if cold else 'warm'} cache)")
+    f_ms, a_ms, wall = run(args.steps, cold)
+    step_ms = f_ms + a_ms
+    ms_per_step = float(np.mean(step_ms))
+    value = nvox / (ms_per_step * 1e-3) / 1e9
+    log(f"bench: fwd {np.mean(f_ms)*1e3:.1f} us, adj {np.mean(a_ms)*1e3:.1f} us, wall {wall:.3f} s")
+    # the other protocol, reported beside
+    f2, a2, _ = run(max(args.steps // 2, 3), not cold)
+    other_value = nvox / (float(np.mean(f2 + a2)) * 1e-3) / 1e9
+
+    # measured streaming-copy ceiling (1 GiB -> 1 GiB... half each way)
+    half = flush_bytes // 2
+    e0, e1 = hip.event(), hip.event()
+    bu.ndl_bench_copy(dflush.ptr, dflush.ptr + half, half, stream)
+    hip.record(e0, stream)
+    for _ in range(5):
+        bu.ndl_bench_copy(dflush.ptr, dflush.ptr + half, half, stream)
+    hip.record(e1, stream)
+    copy_gbps = 5 * 2 * half / (hip.elapsed_ms(e0, e1) * 1e-3) / 1e9
+
+    launch_ms = float(np.mean(np.concatenate([f_ms, a_ms])))
+    alg_bytes = 8 * nvox
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_gauss3d_{n}_f32.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x, axes, taps, borders, nvox)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded counter-based U[0,1) volume, N(0,1) adjoint seed",
+        "config": {
+            "workload": f"separable Gaussian forward+adjoint, f32 {n}^3 volume",
+            "extent": [n, n, n],
+            "sigma": [1.5, 1.5, 3.0],
+            "taps": [len(t) for t in taps],
+            "border": "reflect",
+            "accumulation": "fp32",
+            "step": "ndl_sepconv_forward + ndl_sepconv_adjoint_input",
+            "cache": "cold (1 GiB read flush before each step)" if cold else "warm",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic,
+            "kernel": "fused_tile_kernel (fwd and adjoint launches)",
+            "alg_bytes_per_launch": alg_bytes,
+            "mean_launch_us": round(launch_ms * 1e3, 2),
+        },
+        "cpu_baseline": cpu,
+        ("warm_value" if cold else "cold_value"): round(other_value, 3),
+        "fwd_us": round(float(np.mean(f_ms)) * 1e3, 2),
+        "adj_us": round(float(np.mean(a_ms)) * 1e3, 2),
+        "copy_ceiling_GBps": round(copy_gbps, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(x, axes, taps, borders, nvox):
+    import oracle
+    y = np.empty_like(x)
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    use_ref = oracle.ref_lib is not None
+    log(f"bench: CPU baseline ({'reference build' if use_ref else 'C restatement'}), one core")
+    t0 = time.perf_counter()
+    oracle.sepconv(x, axes, taps, borders, dst=y, use_ref=use_ref)
+    t = time.perf_counter() - t0
+    log(f"bench: CPU forward {t:.2f} s")
+    return {
+        "value": round(nvox / (2 * t) / 1e9, 5),
+        "unit": "Gvoxels/s",
+        "cores": 1,
+        "kind": "reference" if use_ref else "port",
+        "sample": (f"full {round(nvox ** (1 / 3))}^3 volume, one forward of the per-axis convolve() chain "
+                   f"({t:.2f} s), fwd+adj = 2 forwards (self-adjoint reflect Gaussian); host: {cpu_model()}, "
+                   f"nproc {os.cpu_count()}"),
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,821 @@
+// api.hip -- the C ABI of libndl_gpu.so (include/ndl_gpu.h): argument
+// checks, path selection (fused tile kernel or per-axis passes), scratch
+// management and error reporting.  Host code only; kernels live in
+// kernels_*.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ndl_internal.hpp"
+
+#define NDL_API extern "C" __attribute__((visibility("default")))
+
+using namespace ndl;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(NDL_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case NDL_U8: case NDL_I8: return 1;
+    case NDL_U16: case NDL_I16: case NDL_F16: case NDL_BF16: return 2;
+    case NDL_I32: case NDL_F32: return 4;
+    case NDL_F64: return 8;
+    }
+    return 0;
+}
+bool is_float_dt(int dt) { return dt == NDL_F32 || dt == NDL_F64 || dt == NDL_F16 || dt == NDL_BF16; }
+bool is_f32_family(int dt) { return dt == NDL_F32 || dt == NDL_F16 || dt == NDL_BF16; }
+
+int64_t numel(const ndl_tensor* t)
+{
+    int64_t n = 1;
+    for (int d = 0; d < t->ndim; d++) n *= t->extent[d];
+    return n;
+}
+
+int check_tensor(const ndl_tensor* t, const char* name)
+{
+    if (!t) return fail(NDL_ERR_ARG, "%s: null descriptor", name);
+    if (t->ndim < 1 || t->ndim > NDL_MAX_DIM) return fail(NDL_ERR_SHAPE, "%s: ndim %d outside 1..%d", name, t->ndim, NDL_MAX_DIM);
+    if (t->dtype < 0 || t->dtype >= NDL_DTYPE_COUNT) return fail(NDL_ERR_DTYPE, "%s: unknown dtype %d", name, t->dtype);
+    for (int d = 0; d < t->ndim; d++)
+        if (t->extent[d] < 0) return fail(NDL_ERR_SHAPE, "%s: negative extent on dimension %d", name, d);
+    if (numel(t) > 0 && !t->data) return fail(NDL_ERR_ARG, "%s: null data pointer", name);
+    return NDL_OK;
+}
+
+int check_same_shape(const ndl_tensor* a, const ndl_tensor* b, const char* an, const char* bn)
+{
+    if (a->ndim != b->ndim) return fail(NDL_ERR_SHAPE, "%s and %s differ in ndim (%d vs %d)", an, bn, a->ndim, b->ndim);
+    for (int d = 0; d < a->ndim; d++)
+        if (a->extent[d] != b->extent[d])
+            return fail(NDL_ERR_SHAPE, "%s and %s differ in extent on dimension %d", an, bn, d);
+    return NDL_OK;
+}
+
+// byte range [lo, hi) a strided tensor can touch
+void byte_span(const ndl_tensor* t, intptr_t& lo, intptr_t& hi)
+{
+    intptr_t base = (intptr_t)t->data, mn = 0, mx = 0;
+    for (int d = 0; d < t->ndim; d++) {
+        intptr_t reach = (intptr_t)(t->extent[d] - 1) * (intptr_t)t->stride[d];
+        if (reach < 0) mn += reach; else mx += reach;
+    }
+    size_t es = dtype_size(t->dtype);
+    lo = base + mn * (intptr_t)es;
+    hi = base + mx * (intptr_t)es + (intptr_t)es;
+}
+
+int check_no_overlap(const ndl_tensor* a, const ndl_tensor* b)
+{
+    if (numel(a) == 0 || numel(b) == 0) return NDL_OK;
+    intptr_t alo, ahi, blo, bhi;
+    byte_span(a, alo, ahi);
+    byte_span(b, blo, bhi);
+    if (alo < bhi && blo < ahi) return fail(NDL_ERR_ALIAS, "source and destination memory overlap");
+    return NDL_OK;
+}
+
+int check_axes(const ndl_tensor* t, int32_t naxes, const int32_t* axes, const ndl_axis_kernel* k)
+{
+    if (naxes < 1 || naxes > NDL_MAX_DIM) return fail(NDL_ERR_ARG, "naxes %d outside 1..%d", naxes, NDL_MAX_DIM);
+    if (!axes || !k) return fail(NDL_ERR_ARG, "null axes or kernels");
+    for (int i = 0; i < naxes; i++) {
+        int ax = axes[i];
+        if (ax < 0 || ax >= t->ndim) return fail(NDL_ERR_SHAPE, "axis %d outside 0..%d", ax, t->ndim - 1);
+        const ndl_axis_kernel& kk = k[i];
+        if (!kk.taps || kk.count < 1 || kk.count > NDL_MAX_TAPS)
+            return fail(NDL_ERR_ARG, "axis %d: tap count %d outside 1..%d", ax, kk.count, NDL_MAX_TAPS);
+        if (kk.center < 0 || kk.center >= kk.count)
+            return fail(NDL_ERR_ARG, "axis %d: center %d outside 0..%d", ax, kk.center, kk.count - 1);
+        if (kk.border < NDL_CLAMP || kk.border > NDL_REFLECT) return fail(NDL_ERR_ARG, "axis %d: unknown border %d", ax, kk.border);
+        int64_t m = t->extent[ax];
+        // single-fold wrap/reflect maps (mathHelpers.h:23-24) are only defined
+        // for taps reaching at most one extent past either edge
+        if (kk.border != NDL_CLAMP && m > 0 && (kk.center > m || kk.count - 1 - kk.center > m))
+            return fail(NDL_ERR_RADIUS, "axis %d: taps reach %d/%d past the edges of an extent of %lld", ax,
+                        kk.center, kk.count - 1 - kk.center, (long long)m);
+    }
+    return NDL_OK;
+}
+
+// ------------------------------------------------------------ scratch memory
+struct Scratch {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    ~Scratch()
+    {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
+
+int alloc_scratch(Scratch& sc, size_t bytes, hipStream_t s)
+{
+    sc.s = s;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMallocAsync(&sc.p, bytes, s);
+    if (e != hipSuccess) {
+        sc.p = nullptr;
+        return fail(NDL_ERR_NOMEM, "scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    return NDL_OK;
+}
+
+ndl_tensor dense_like(const ndl_tensor* t, void* data, int dtype)
+{
+    ndl_tensor r = *t;
+    r.data = data;
+    r.dtype = dtype;
+    int64_t st = 1;
+    for (int d = 0; d < t->ndim; d++) {
+        r.stride[d] = st;
+        st *= t->extent[d];
+    }
+    return r;
+}
+
+// -------------------------------------------------------- generic axis pass
+int run_axis_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& k,
+                  int accum, int rounding, int mode, const void* table, int EL, int ER, hipStream_t s)
+{
+    int64_t n = numel(in);
+    if (n == 0) return NDL_OK;
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    AxisPassArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = in->data;
+    a.out = out->data;
+    a.m = in->extent[axis];
+    a.in_as = in->stride[axis];
+    // iterate dimensions fastest-in-memory first so a wavefront reads
+    // contiguous addresses for every tap
+    int order[NDL_MAX_DIM];
+    for (int d = 0; d < in->ndim; d++) order[d] = d;
+    std::stable_sort(order, order + in->ndim, [&](int x, int y) {
+        return std::llabs(in->stride[x]) < std::llabs(in->stride[y]);
+    });
+    a.nd = in->ndim;
+    for (int i = 0; i < in->ndim; i++) {
+        int d = order[i];
+        a.ext[i] = in->extent[d];
+        a.ist[i] = in->stride[d];
+        a.ost[i] = out->stride[d];
+        a.fdiv[i].init((uint32_t)in->extent[d]);
+        if (d == axis) a.axis_pos = i;
+    }
+    a.total = (uint32_t)n;
+    a.K = k.count;
+    a.center = k.center;
+    a.border = k.border;
+    a.rounding = rounding;
+    a.mode = mode;
+    a.table = table;
+    a.EL = EL;
+    a.ER = ER;
+    for (int t = 0; t < k.count; t++) {
+        a.taps_d[t] = k.taps[t];
+        a.taps_f[t] = (float)k.taps[t];
+    }
+    hipError_t e = launch_axis_pass(a, in->dtype, out->dtype, accum, s);
+    if (e != hipSuccess) return hip_fail(e, "axis pass launch");
+    return NDL_OK;
+}
+
+// per-axis chain through dense accumulator-typed intermediates
+int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+                      const ndl_axis_kernel* k, int accum, int rounding, hipStream_t s)
+{
+    const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
+    Scratch buf[2];
+    ndl_tensor mid[2];
+    if (naxes > 1) {
+        size_t bytes = (size_t)numel(src) * dtype_size(acc_dt);
+        for (int i = 0; i < 2 && i < naxes - 1; i++) {
+            int rc = alloc_scratch(buf[i], bytes, s);
+            if (rc) return rc;
+            mid[i] = dense_like(src, buf[i].p, acc_dt);
+        }
+    }
+    const ndl_tensor* cur = src;
+    for (int i = 0; i < naxes; i++) {
+        bool last = i == naxes - 1;
+        const ndl_tensor* out = last ? dst : &mid[i & 1];
+        int rc = run_axis_pass(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC,
+                               PASS_GATHER, nullptr, 0, 0, s);
+        if (rc) return rc;
+        cur = out;
+    }
+    return NDL_OK;
+}
+
+// ------------------------------------------------------------ fused path
+bool taps_integral(const ndl_axis_kernel& k, double& abs_sum)
+{
+    abs_sum = 0;
+    for (int t = 0; t < k.count; t++) {
+        double w = k.taps[t];
+        if (!(std::fabs(w) <= 32767.0) || w != std::floor(w)) return false;
+        abs_sum += std::fabs(w);
+    }
+    return true;
+}
+
+int env_int(const char* name, int dflt)
+{
+    const char* v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+// Fills FusedArgs for src/dst if the fused kernel applies; returns false
+// otherwise (the caller then runs the per-axis chain).
+bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+                const ndl_axis_kernel* k, int accum, FusedArgs& fa, FusedConfig& cfg, bool& int_acc)
+{
+    if (env_int("NDL_DISABLE_FUSED", 0)) return false;
+    if (src->dtype != dst->dtype || naxes > 3) return false;
+    const int dt = src->dtype;
+    int_acc = false;
+    if (is_f32_family(dt)) {
+        if (accum != NDL_ACCUM_F32) return false;
+    } else if (dt == NDL_U8) {
+        // exact integer arithmetic: every partial sum must fit in int32
+        double bound = dt == NDL_U8 ? 255.0 : 65535.0;
+        for (int i = 0; i < naxes; i++) {
+            double as;
+            if (!taps_integral(k[i], as)) return false;
+            bound *= as;
+        }
+        if (bound >= 2147483647.0) return false;
+        int_acc = true;
+    } else {
+        return false;
+    }
+    const int nd = src->ndim;
+    int role[NDL_MAX_DIM]; // -1 batch, 0 x, 1 y, 2 z
+    for (int d = 0; d < nd; d++) role[d] = -1;
+    int conv_of_dim[NDL_MAX_DIM];
+    for (int d = 0; d < nd; d++) conv_of_dim[d] = -1;
+    for (int i = 0; i < naxes; i++) {
+        if (k[i].count > 25) return false;
+        if (conv_of_dim[axes[i]] >= 0) return false; // an axis listed twice
+        conv_of_dim[axes[i]] = i;
+    }
+    int xd = -1;
+    for (int d = 0; d < nd; d++)
+        if (src->stride[d] == 1 && dst->stride[d] == 1 && src->extent[d] > 1) { xd = d; break; }
+    if (xd < 0) return false;
+    role[xd] = 0;
+    int yd = -1, zd = -1;
+    for (int i = 0; i < naxes; i++) {
+        int d = axes[i];
+        if (d == xd) continue;
+        if (yd < 0) yd = d;
+        else if (zd < 0) zd = d;
+        else return false;
+    }
+    if (yd >= 0) role[yd] = 1;
+    if (zd >= 0) role[zd] = 2;
+    memset(&fa, 0, sizeof fa);
+    fa.src = src->data;
+    fa.dst = dst->data;
+    fa.X = (int32_t)src->extent[xd];
+    fa.Y = yd >= 0 ? (int32_t)src->extent[yd] : 1;
+    fa.Z = zd >= 0 ? (int32_t)src->extent[zd] : 1;
+    fa.sy = yd >= 0 ? src->stride[yd] : 0;
+    fa.dy = yd >= 0 ? dst->stride[yd] : 0;
+    fa.sz = zd >= 0 ? src->stride[zd] : 0;
+    fa.dz = zd >= 0 ? dst->stride[zd] : 0;
+    if (src->extent[xd] > 0x7fffffff || fa.Y <= 0 || fa.Z <= 0) return false;
+    // in-slice offsets are 32-bit
+    if (std::llabs(fa.sy) * (int64_t)(fa.Y - 1) + fa.X >= 0x7fffffffLL) return false;
+    if (std::llabs(fa.dy) * (int64_t)(fa.Y - 1) + fa.X >= 0x7fffffffLL) return false;
+    int64_t batch = 1;
+    fa.nb = 0;
+    for (int d = 0; d < nd; d++) {
+        if (role[d] >= 0) continue;
+        if (src->extent[d] == 1) continue;
+        fa.bext[fa.nb] = src->extent[d];
+        fa.sb[fa.nb] = src->stride[d];
+        fa.db[fa.nb] = dst->stride[d];
+        fa.bdiv[fa.nb].init((uint32_t)src->extent[d]);
+        batch *= src->extent[d];
+        fa.nb++;
+    }
+    auto set_axis = [&](int d, int32_t& kk, int32_t& c, int32_t& b, float* wf, int32_t* wi, bool lag) {
+        if (d < 0 || conv_of_dim[d] < 0) {
+            kk = 1; c = 0; b = NDL_CLAMP; wf[0] = 1.0f; wi[0] = 1;
+            return;
+        }
+        const ndl_axis_kernel& ak = k[conv_of_dim[d]];
+        kk = ak.count; c = ak.center; b = ak.border;
+        for (int t = 0; t < ak.count; t++) {
+            int idx = lag ? ak.count - 1 - t : t;
+            wf[idx] = (float)ak.taps[t];
+            wi[idx] = (int32_t)ak.taps[t];
+        }
+    };
+    set_axis(xd, fa.kx, fa.cx, fa.bx, fa.wx, fa.iwx, false);
+    set_axis(yd, fa.ky, fa.cy, fa.by, fa.wy, fa.iwy, false);
+    set_axis(zd, fa.kz, fa.cz, fa.bz, fa.wzlag, fa.iwzlag, true);
+    cfg.kmax = 25;
+    cfg.tx = 64;
+    cfg.ty = 16;
+    cfg.ring = fused_ring_for(fa.kz);
+    if (cfg.ring < 0) return false;
+    fa.tiles_x = (fa.X + cfg.tx - 1) / cfg.tx;
+    fa.tiles_y = (fa.Y + cfg.ty - 1) / cfg.ty;
+    int64_t xy_tiles = (int64_t)fa.tiles_x * fa.tiles_y * batch;
+    // z chunking: enough workgroups to cover the chip, long marches otherwise
+    int target = env_int("NDL_FUSED_TARGET_WG", 512);
+    int tz = fa.Z;
+    if (fa.Z > 1 && xy_tiles < target) {
+        int64_t chunks = (target + xy_tiles - 1) / xy_tiles;
+        tz = (int)((fa.Z + chunks - 1) / chunks);
+        tz = std::max(tz, std::max(16, 2 * fa.kz));
+        tz = std::min(tz, fa.Z);
+    }
+    int tz_env = env_int("NDL_FUSED_TZ", 0);
+    if (tz_env > 0) tz = std::min(tz_env, fa.Z);
+    fa.tz = tz;
+    fa.tiles_z = (fa.Z + tz - 1) / tz;
+    int64_t ntiles = xy_tiles * fa.tiles_z;
+    if (ntiles <= 0 || ntiles > 0x7fffffffLL || batch > 0xffffffffLL) return false;
+    fa.ntiles = (uint32_t)ntiles;
+    fa.fdx.init((uint32_t)fa.tiles_x);
+    fa.fdy.init((uint32_t)fa.tiles_y);
+    fa.fdz.init((uint32_t)fa.tiles_z);
+    fa.xcd_remap = (ntiles % 8 == 0) && env_int("NDL_XCD_REMAP", 1);
+    return true;
+}
+
+int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+                 const ndl_axis_kernel* k, int32_t accum, int32_t rounding, hipStream_t s)
+{
+    FusedArgs fa;
+    FusedConfig cfg;
+    bool int_acc = false;
+    if (plan_fused(src, dst, naxes, axes, k, accum, fa, cfg, int_acc)) {
+        hipError_t e = launch_fused(fa, cfg, src->dtype, int_acc, s);
+        if (e == hipSuccess) return NDL_OK;
+        if (e != hipErrorInvalidValue) return hip_fail(e, "fused launch");
+        // no instantiation for this combination: fall through to the chain
+    }
+    return run_generic_chain(src, dst, naxes, axes, k, accum, rounding, s);
+}
+
+int check_forward_args(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+                       const ndl_axis_kernel* k, int32_t accum, int32_t rounding)
+{
+    int rc;
+    if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
+    if ((rc = check_same_shape(src, dst, "src", "dst"))) return rc;
+    if ((rc = check_axes(src, naxes, axes, k))) return rc;
+    if (accum != NDL_ACCUM_REF && accum != NDL_ACCUM_F32) return fail(NDL_ERR_ARG, "unknown accumulation mode %d", accum);
+    if (rounding != NDL_ROUND_TRUNC && rounding != NDL_ROUND_NEAREST) return fail(NDL_ERR_ARG, "unknown rounding %d", rounding);
+    if (accum == NDL_ACCUM_F32 && !(is_f32_family(src->dtype) && is_f32_family(dst->dtype)))
+        return fail(NDL_ERR_DTYPE, "FP32 accumulation needs f32/f16/bf16 source and destination");
+    if ((rc = check_no_overlap(src, dst))) return rc;
+    return NDL_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+// Transposed axis kernel: wrap transposes to wrap with flipped taps; reflect
+// with symmetric taps and a centred odd kernel is self-adjoint; clamp and
+// asymmetric reflect need per-edge weight rows (PASS_TABLE).
+struct AdjAxis {
+    std::vector<double> taps; // flipped
+    ndl_axis_kernel k;
+    bool table;
+    std::vector<double> rows; // (EL+ER) x K
+    int EL = 0, ER = 0;
+};
+
+bool symmetric(const ndl_axis_kernel& k)
+{
+    if (k.count % 2 == 0 || k.center != k.count / 2) return false;
+    for (int t = 0; t < k.count; t++)
+        if (k.taps[t] != k.taps[k.count - 1 - t]) return false;
+    return true;
+}
+
+void build_adjoint_axis(const ndl_axis_kernel& k, int64_t m, AdjAxis& out)
+{
+    const int K = k.count, c = k.center;
+    out.taps.resize(K);
+    for (int s = 0; s < K; s++) out.taps[s] = k.taps[K - 1 - s];
+    out.k.taps = out.taps.data();
+    out.k.count = K;
+    out.k.center = K - 1 - c;
+    out.k.border = k.border;
+    out.table = !(k.border == NDL_WRAP || (k.border == NDL_REFLECT && symmetric(k)));
+    if (!out.table) return;
+    // row i of A^T in window form: out[i] = sum_s V[i][s] * g[i + s - c'],
+    // V[i][s] = sum_t w[t] [B(j + t - c) == i], j = i + s - c'.  Rows farther
+    // than K from either edge equal the flipped band.
+    const int cp = K - 1 - c;
+    out.EL = (int)std::min<int64_t>(K, m);
+    out.ER = (int)std::min<int64_t>(K, m - out.EL);
+    out.rows.assign((size_t)(out.EL + out.ER) * K, 0.0);
+    for (int r = 0; r < out.EL + out.ER; r++) {
+        int64_t i = r < out.EL ? r : m - out.ER + (r - out.EL);
+        for (int s = 0; s < K; s++) {
+            int64_t j = i + s - cp;
+            double v = 0;
+            if (j >= 0 && j < m)
+                for (int t = 0; t < K; t++)
+                    if (border_map(k.border, m, j + t - c) == i) v += k.taps[t];
+            out.rows[(size_t)r * K + s] = v;
+        }
+    }
+}
+
+int adjoint_input_impl(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes, const int32_t* axes,
+                       const ndl_axis_kernel* k, int32_t accum, hipStream_t s)
+{
+    std::vector<AdjAxis> adj(naxes);
+    bool any_table = false;
+    for (int i = 0; i < naxes; i++) {
+        build_adjoint_axis(k[i], g->extent[axes[i]], adj[i]);
+        any_table |= adj[i].table;
+    }
+    // A = A_{n-1}..A_0, so A^T applies A_{n-1}^T first
+    std::vector<int32_t> rax(naxes);
+    std::vector<ndl_axis_kernel> rk(naxes);
+    for (int i = 0; i < naxes; i++) {
+        rax[i] = axes[naxes - 1 - i];
+        rk[i] = adj[naxes - 1 - i].k;
+    }
+    if (!any_table) return forward_impl(g, gx, naxes, rax.data(), rk.data(), accum, NDL_ROUND_TRUNC, s);
+
+    const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
+    const size_t acc_sz = dtype_size(acc_dt);
+    Scratch buf[2], tabs;
+    ndl_tensor mid[2];
+    size_t tab_bytes = 0;
+    for (auto& a : adj) tab_bytes += a.rows.size() * acc_sz;
+    int rc = alloc_scratch(tabs, tab_bytes, s);
+    if (rc) return rc;
+    std::vector<unsigned char> host_tab(std::max<size_t>(tab_bytes, 1));
+    std::vector<size_t> tab_off(naxes, 0);
+    size_t off = 0;
+    for (int i = 0; i < naxes; i++) {
+        tab_off[i] = off;
+        for (size_t r = 0; r < adj[i].rows.size(); r++) {
+            if (acc_dt == NDL_F64) ((double*)(host_tab.data() + off))[r] = adj[i].rows[r];
+            else ((float*)(host_tab.data() + off))[r] = (float)adj[i].rows[r];
+        }
+        off += adj[i].rows.size() * acc_sz;
+    }
+    if (tab_bytes) {
+        hipError_t e = hipMemcpyAsync(tabs.p, host_tab.data(), tab_bytes, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "edge-table upload");
+        // the host buffer dies with this call: make sure the copy has read it
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "edge-table upload");
+    }
+    if (naxes > 1) {
+        size_t bytes = (size_t)numel(g) * acc_sz;
+        for (int i = 0; i < 2 && i < naxes - 1; i++) {
+            if ((rc = alloc_scratch(buf[i], bytes, s))) return rc;
+            mid[i] = dense_like(g, buf[i].p, acc_dt);
+        }
+    }
+    const ndl_tensor* cur = g;
+    for (int i = 0; i < naxes; i++) {
+        const int src_i = naxes - 1 - i;
+        bool last = i == naxes - 1;
+        const ndl_tensor* out = last ? gx : &mid[i & 1];
+        const AdjAxis& a = adj[src_i];
+        rc = run_axis_pass(cur, out, axes[src_i], a.k, accum, NDL_ROUND_TRUNC, a.table ? PASS_TABLE : PASS_GATHER,
+                           a.table ? (const void*)((unsigned char*)tabs.p + tab_off[src_i]) : nullptr, a.EL, a.ER, s);
+        if (rc) return rc;
+        cur = out;
+    }
+    return NDL_OK;
+}
+
+int adjoint_taps_impl(const ndl_tensor* x, const ndl_tensor* g, int32_t naxes, const int32_t* axes,
+                      const ndl_axis_kernel* k, double* grad_taps, int32_t accum, hipStream_t s)
+{
+    const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
+    const size_t acc_sz = dtype_size(acc_dt);
+    int64_t n = numel(x);
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    Scratch ubuf, part;
+    int rc = alloc_scratch(ubuf, (size_t)n * acc_sz, s);
+    if (rc) return rc;
+    ndl_tensor u = dense_like(x, ubuf.p, acc_dt);
+    const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+    int maxk = 0;
+    for (int i = 0; i < naxes; i++) maxk = std::max(maxk, k[i].count);
+    if ((rc = alloc_scratch(part, (size_t)nblocks * maxk * sizeof(double), s))) return rc;
+    static const double one = 1.0;
+    int off = 0;
+    for (int a = 0; a < naxes; a++) {
+        // U_a = x through every other axis (an identity pass when there is none)
+        std::vector<int32_t> oax;
+        std::vector<ndl_axis_kernel> ok;
+        for (int b = 0; b < naxes; b++)
+            if (b != a) { oax.push_back(axes[b]); ok.push_back(k[b]); }
+        if (oax.empty()) {
+            oax.push_back(axes[a]);
+            ok.push_back(ndl_axis_kernel{&one, 1, 0, NDL_CLAMP});
+        }
+        if ((rc = run_generic_chain(x, &u, (int)oax.size(), oax.data(), ok.data(), accum, NDL_ROUND_TRUNC, s))) return rc;
+        TapGradArgs ta;
+        memset(&ta, 0, sizeof ta);
+        ta.g = g->data;
+        ta.u = u.data;
+        ta.nd = x->ndim;
+        int order[NDL_MAX_DIM];
+        for (int d = 0; d < x->ndim; d++) order[d] = d;
+        std::stable_sort(order, order + x->ndim,
+                         [&](int p, int q) { return std::llabs(g->stride[p]) < std::llabs(g->stride[q]); });
+        for (int i = 0; i < x->ndim; i++) {
+            int d = order[i];
+            ta.ext[i] = x->extent[d];
+            ta.gst[i] = g->stride[d];
+            ta.ust[i] = u.stride[d];
+            ta.fdiv[i].init((uint32_t)x->extent[d]);
+            if (d == axes[a]) ta.axis_pos = i;
+        }
+        ta.total = (uint32_t)n;
+        ta.m = x->extent[axes[a]];
+        ta.u_as = u.stride[axes[a]];
+        ta.K = k[a].count;
+        ta.center = k[a].center;
+        ta.border = k[a].border;
+        ta.partial = (double*)part.p;
+        ta.nblocks = nblocks;
+        hipError_t e = launch_tapgrad_partial(ta, g->dtype, acc_dt == NDL_F64, s);
+        if (e != hipSuccess) return hip_fail(e, "tap-gradient launch");
+        e = launch_tapgrad_final((const double*)part.p, nblocks, k[a].count, grad_taps + off, s);
+        if (e != hipSuccess) return hip_fail(e, "tap-gradient launch");
+        off += k[a].count;
+    }
+    return NDL_OK;
+}
+
+} // namespace
+
+// ==================================================================== ABI
+
+NDL_API int ndl_sepconv_forward(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+                                const ndl_axis_kernel* k, int32_t accum, int32_t rounding, void* stream)
+{
+    int rc = check_forward_args(src, dst, naxes, axes, k, accum, rounding);
+    if (rc) return rc;
+    if (numel(src) == 0) return NDL_OK;
+    return forward_impl(src, dst, naxes, axes, k, accum, rounding, (hipStream_t)stream);
+}
+
+NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel,
+                            const int64_t* kextent, int32_t border, void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
+    if ((rc = check_same_shape(src, dst, "src", "dst"))) return rc;
+    if (!kernel || !kextent) return fail(NDL_ERR_ARG, "null kernel");
+    if (border < NDL_CLAMP || border > NDL_REFLECT) return fail(NDL_ERR_ARG, "unknown border %d", border);
+    if ((rc = check_no_overlap(src, dst))) return rc;
+    const int nd = src->ndim;
+    int64_t kn = 1;
+    for (int d = 0; d < nd; d++) {
+        if (kextent[d] < 1) return fail(NDL_ERR_SHAPE, "kernel extent %lld on dimension %d", (long long)kextent[d], d);
+        int64_t c = kextent[d] / 2, m = src->extent[d];
+        if (border != NDL_CLAMP && m > 0 && (c > m || kextent[d] - 1 - c > m))
+            return fail(NDL_ERR_RADIUS, "kernel reaches past one extent on dimension %d", d);
+        kn *= kextent[d];
+    }
+    int64_t n = numel(src);
+    if (n == 0) return NDL_OK;
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    // detail::kernelIncludedTaps: kernel coordinates dimension 0 fastest,
+    // nonzero taps only, offsets relative to the centre
+    std::vector<int32_t> offs;
+    std::vector<double> vals;
+    for (int64_t lin = 0; lin < kn; lin++) {
+        if (kernel[lin] == 0.0) continue;
+        int64_t r = lin;
+        for (int d = 0; d < nd; d++) {
+            offs.push_back((int32_t)(r % kextent[d] - kextent[d] / 2));
+            r /= kextent[d];
+        }
+        vals.push_back(kernel[lin]);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    Scratch tb;
+    size_t ob = offs.size() * sizeof(int32_t), vb = vals.size() * sizeof(double);
+    if ((rc = alloc_scratch(tb, ob + vb + 16, s))) return rc;
+    char* dev = (char*)tb.p;
+    size_t voff = (ob + 15) & ~(size_t)15;
+    std::vector<char> host(voff + vb + 1);
+    memcpy(host.data(), offs.data(), ob);
+    memcpy(host.data() + voff, vals.data(), vb);
+    hipError_t e = hipMemcpyAsync(dev, host.data(), voff + vb, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "kernel tap upload");
+    NdArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = src->data;
+    a.out = dst->data;
+    a.nd = nd;
+    for (int d = 0; d < nd; d++) {
+        a.ext[d] = src->extent[d];
+        a.ist[d] = src->stride[d];
+        a.ost[d] = dst->stride[d];
+        a.fdiv[d].init((uint32_t)src->extent[d]);
+    }
+    a.total = (uint32_t)n;
+    a.border = border;
+    a.ntaps = (int32_t)vals.size();
+    a.tap_off = (const int32_t*)dev;
+    a.tap_val = (const double*)(dev + voff);
+    e = launch_convolve_nd(a, src->dtype, dst->dtype, s);
+    if (e != hipSuccess) return hip_fail(e, "convolve_nd launch");
+    return NDL_OK;
+}
+
+NDL_API int ndl_sepconv_adjoint_input(const ndl_tensor* grad_out, const ndl_tensor* grad_in, int32_t naxes,
+                                      const int32_t* axes, const ndl_axis_kernel* k, int32_t accum, void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(grad_out, "grad_out")) || (rc = check_tensor(grad_in, "grad_in"))) return rc;
+    if ((rc = check_same_shape(grad_out, grad_in, "grad_out", "grad_in"))) return rc;
+    if ((rc = check_axes(grad_out, naxes, axes, k))) return rc;
+    if (!is_float_dt(grad_out->dtype) || !is_float_dt(grad_in->dtype))
+        return fail(NDL_ERR_DTYPE, "adjoint fields must be floating point");
+    if (accum != NDL_ACCUM_REF && accum != NDL_ACCUM_F32) return fail(NDL_ERR_ARG, "unknown accumulation mode %d", accum);
+    if (accum == NDL_ACCUM_F32 && !(is_f32_family(grad_out->dtype) && is_f32_family(grad_in->dtype)))
+        return fail(NDL_ERR_DTYPE, "FP32 accumulation needs f32/f16/bf16 fields");
+    if ((rc = check_no_overlap(grad_out, grad_in))) return rc;
+    if (numel(grad_out) == 0) return NDL_OK;
+    return adjoint_input_impl(grad_out, grad_in, naxes, axes, k, accum, (hipStream_t)stream);
+}
+
+NDL_API int ndl_sepconv_adjoint_taps(const ndl_tensor* x, const ndl_tensor* grad_out, int32_t naxes,
+                                     const int32_t* axes, const ndl_axis_kernel* k, double* grad_taps,
+                                     int32_t accum, void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(x, "x")) || (rc = check_tensor(grad_out, "grad_out"))) return rc;
+    if ((rc = check_same_shape(x, grad_out, "x", "grad_out"))) return rc;
+    if ((rc = check_axes(x, naxes, axes, k))) return rc;
+    if (!grad_taps) return fail(NDL_ERR_ARG, "null grad_taps");
+    if (!is_float_dt(grad_out->dtype)) return fail(NDL_ERR_DTYPE, "grad_out must be floating point");
+    if (accum != NDL_ACCUM_REF && accum != NDL_ACCUM_F32) return fail(NDL_ERR_ARG, "unknown accumulation mode %d", accum);
+    if (accum == NDL_ACCUM_F32 && !is_f32_family(x->dtype))
+        return fail(NDL_ERR_DTYPE, "FP32 accumulation needs an f32/f16/bf16 input");
+    hipStream_t s = (hipStream_t)stream;
+    if (numel(x) == 0) {
+        int total = 0;
+        for (int i = 0; i < naxes; i++) total += k[i].count;
+        hipError_t e = hipMemsetAsync(grad_taps, 0, sizeof(double) * total, s);
+        return e == hipSuccess ? NDL_OK : hip_fail(e, "grad_taps clear");
+    }
+    return adjoint_taps_impl(x, grad_out, naxes, axes, k, grad_taps, accum, s);
+}
+
+NDL_API int ndl_gaussian_taps(double sigma, double* taps, int32_t cap)
+{
+    if (!(sigma > 0)) return fail(NDL_ERR_ARG, "sigma must be > 0");
+    if (!taps) return fail(NDL_ERR_ARG, "null taps");
+    // _kernelSize: (int)_ceil(3.0f * sigma) (mathHelpers.h:17-18)
+    double num = 3.0 * sigma;
+    if (!(num < 1e6)) return fail(NDL_ERR_ARG, "sigma too large");
+    int t = (int)num;
+    int radius = ((double)t == num) ? t : t + (num > 0 ? 1 : 0);
+    int width = 2 * radius + 1;
+    if (width > cap) return fail(NDL_ERR_ARG, "need %d taps, capacity %d", width, cap);
+    double wsum = 0;
+    for (int i = 0; i < width; i++) {
+        double d = i - radius;
+        taps[i] = std::exp(-(d * d) / (2 * sigma * sigma));
+        wsum += taps[i];
+    }
+    for (int i = 0; i < width; i++) taps[i] /= wsum;
+    return width;
+}
+
+// ---------------------------------------------------------------- helpers
+NDL_API int ndl_malloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(NDL_ERR_ARG, "null ptr");
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(NDL_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return NDL_OK;
+}
+NDL_API int ndl_free(void* ptr)
+{
+    hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "hipFree");
+}
+NDL_API int ndl_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "hipMemcpyAsync H2D");
+}
+NDL_API int ndl_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "hipMemcpyAsync D2H");
+}
+NDL_API int ndl_memset(void* dst, int value, size_t bytes, void* stream)
+{
+    hipError_t e = hipMemsetAsync(dst, value, bytes, (hipStream_t)stream);
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "hipMemsetAsync");
+}
+NDL_API int ndl_stream_synchronize(void* stream)
+{
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "hipStreamSynchronize");
+}
+NDL_API int ndl_device_count(int* count)
+{
+    if (!count) return fail(NDL_ERR_ARG, "null count");
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    return NDL_OK;
+}
+NDL_API const char* ndl_last_error(void) { return g_err.c_str(); }
+NDL_API int ndl_version(void) { return 100; }
+
+// ---------------------------------------------------------------- batched
+struct ndl_plan {
+    std::vector<ndl_batch_item> items;
+    std::vector<std::vector<double>> taps; // owned copies, NDL_MAX_DIM per item
+    int32_t accum = NDL_ACCUM_F32;
+};
+
+NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accum, ndl_plan** plan)
+{
+    if (!plan) return fail(NDL_ERR_ARG, "null plan");
+    *plan = nullptr;
+    if (n < 0 || (n > 0 && !items)) return fail(NDL_ERR_ARG, "bad item list");
+    if (accum != NDL_ACCUM_REF && accum != NDL_ACCUM_F32) return fail(NDL_ERR_ARG, "unknown accumulation mode %d", accum);
+    ndl_plan* p = new ndl_plan();
+    p->accum = accum;
+    p->items.assign(items, items + n);
+    p->taps.resize((size_t)n * NDL_MAX_DIM);
+    for (int i = 0; i < n; i++) {
+        ndl_batch_item& it = p->items[i];
+        int rc = check_forward_args(&it.src, &it.dst, it.naxes, it.axes, it.k, accum, NDL_ROUND_TRUNC);
+        if (rc) {
+            delete p;
+            return fail(rc, "item %d: %s", i, g_err.c_str());
+        }
+        for (int a = 0; a < it.naxes; a++) {
+            auto& v = p->taps[(size_t)i * NDL_MAX_DIM + a];
+            v.assign(it.k[a].taps, it.k[a].taps + it.k[a].count);
+            it.k[a].taps = v.data();
+        }
+    }
+    *plan = p;
+    return NDL_OK;
+}
+
+NDL_API int ndl_plan_execute(ndl_plan* plan, void* stream)
+{
+    if (!plan) return fail(NDL_ERR_ARG, "null plan");
+    for (size_t i = 0; i < plan->items.size(); i++) {
+        const ndl_batch_item& it = plan->items[i];
+        if (numel(&it.src) == 0) continue;
+        int rc = forward_impl(&it.src, &it.dst, it.naxes, it.axes, it.k, plan->accum, NDL_ROUND_TRUNC,
+                              (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return NDL_OK;
+}
+
+NDL_API int ndl_plan_destroy(ndl_plan* plan)
+{
+    delete plan;
+    return NDL_OK;
+}

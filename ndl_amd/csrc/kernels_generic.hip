@@ -1,0 +1,221 @@
+// kernels_generic.hip -- the always-correct kernels: one convolved axis per
+// launch over any strides, element types and borders, plus the dense N-D
+// kernel and the tap-gradient reduction.
+//
+// Semantics follow processing/convolution.h:52-71 per output: taps ascending,
+// exactly-zero taps skipped (detail.h:164-174), border resolved per tap
+// (detail.h:178-194).  NDL_ACCUM_REF accumulates double(src)*double(tap) with
+// every product and sum rounded separately (this file is built with
+// -ffp-contract=off), so results are bit-identical to the reference.
+#include "ndl_internal.hpp"
+
+namespace ndl {
+
+template <class Acc> __device__ __forceinline__ Acc mac(Acc acc, Acc v, Acc w);
+template <> __device__ __forceinline__ double mac<double>(double acc, double v, double w)
+{
+    return acc + v * w; // two roundings, as the reference's `total += src * k`
+}
+template <> __device__ __forceinline__ float mac<float>(float acc, float v, float w)
+{
+    return __builtin_fmaf(v, w, acc);
+}
+
+template <class Acc> __device__ __forceinline__ Acc tap_of(const AxisPassArgs& a, int t);
+template <> __device__ __forceinline__ double tap_of<double>(const AxisPassArgs& a, int t) { return a.taps_d[t]; }
+template <> __device__ __forceinline__ float tap_of<float>(const AxisPassArgs& a, int t) { return a.taps_f[t]; }
+
+template <class T, class Acc> __device__ __forceinline__ void store_acc(T* p, Acc v, int rounding)
+{
+    if constexpr (sizeof(Acc) == 8) store_from_f64(p, v, rounding);
+    else store_from_f32(p, v);
+}
+
+// ---------------------------------------------------------------- axis pass
+template <class TIn, class TOut, class Acc>
+__global__ __launch_bounds__(256) void axis_pass_kernel(const AxisPassArgs a)
+{
+    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.total) return;
+    int64_t ioff = 0, ooff = 0, j = 0;
+    uint32_t rem = idx;
+    for (int d = 0; d < a.nd; d++) {
+        uint32_t q = a.fdiv[d].div(rem);
+        int64_t c = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+        rem = q;
+        ioff += c * a.ist[d];
+        ooff += c * a.ost[d];
+        if (d == a.axis_pos) j = c;
+    }
+    const TIn* in = (const TIn*)a.in + (ioff - j * a.in_as);
+    Acc total = 0;
+    if (a.mode == PASS_GATHER) {
+        for (int t = 0; t < a.K; t++) {
+            Acc w = tap_of<Acc>(a, t);
+            if (w == (Acc)0) continue; // uniform: zero taps are not visited
+            int64_t x = j + t - a.center;
+            if (x < 0 || x >= a.m) x = border_map(a.border, a.m, x);
+            total = mac<Acc>(total, load_acc<Acc>(in + x * a.in_as), w);
+        }
+    } else {
+        const Acc* row = nullptr;
+        if (j < a.EL) row = (const Acc*)a.table + j * a.K;
+        else if (j >= a.m - a.ER) row = (const Acc*)a.table + (a.EL + (j - (a.m - a.ER))) * a.K;
+        for (int t = 0; t < a.K; t++) {
+            Acc w = row ? row[t] : tap_of<Acc>(a, t);
+            int64_t x = j + t - a.center;
+            if (x < 0 || x >= a.m || w == (Acc)0) continue; // zero outside the axis
+            total = mac<Acc>(total, load_acc<Acc>(in + x * a.in_as), w);
+        }
+    }
+    store_acc<TOut, Acc>((TOut*)a.out + ooff, total, a.rounding);
+}
+
+// --------------------------------------------------------- dtype dispatch
+// Calls f(T{}) with the storage type of dtype code dt.
+template <class F> hipError_t with_dtype(int dt, F&& f)
+{
+    switch (dt) {
+    case NDL_U8: return f(uint8_t{});
+    case NDL_I8: return f(int8_t{});
+    case NDL_U16: return f(uint16_t{});
+    case NDL_I16: return f(int16_t{});
+    case NDL_I32: return f(int32_t{});
+    case NDL_F32: return f(float{});
+    case NDL_F64: return f(double{});
+    case NDL_F16: return f(f16_t{});
+    case NDL_BF16: return f(bf16_t{});
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_axis_pass(const AxisPassArgs& a, int in_dt, int out_dt, int accum, hipStream_t s)
+{
+    if (a.total == 0) return hipSuccess;
+    uint32_t blocks = (a.total + 255u) / 256u;
+    auto go = [&](auto acc_tag, auto tin) {
+        using Acc = decltype(acc_tag);
+        using TIn = decltype(tin);
+        return with_dtype(out_dt, [&](auto tout) {
+            using TOut = decltype(tout);
+            hipLaunchKernelGGL((axis_pass_kernel<TIn, TOut, Acc>), dim3(blocks), dim3(256), 0, s, a);
+            return hipGetLastError();
+        });
+    };
+    if (accum == NDL_ACCUM_REF) return with_dtype(in_dt, [&](auto tin) { return go(double{}, tin); });
+    // FP32 accumulation is offered for float storage only
+    switch (in_dt) {
+    case NDL_F32: return go(float{}, float{});
+    case NDL_F16: return go(float{}, f16_t{});
+    case NDL_BF16: return go(float{}, bf16_t{});
+    }
+    return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------ dense N-D
+template <class TIn, class TOut>
+__global__ __launch_bounds__(256) void convolve_nd_kernel(const NdArgs a)
+{
+    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.total) return;
+    int64_t coord[NDL_MAX_DIM];
+    int64_t ooff = 0;
+    uint32_t rem = idx;
+    for (int d = 0; d < a.nd; d++) {
+        uint32_t q = a.fdiv[d].div(rem);
+        coord[d] = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+        rem = q;
+        ooff += coord[d] * a.ost[d];
+    }
+    double total = 0;
+    for (int t = 0; t < a.ntaps; t++) {
+        int64_t off = 0;
+        for (int d = 0; d < a.nd; d++) {
+            int64_t x = coord[d] + a.tap_off[t * a.nd + d];
+            if (x < 0 || x >= a.ext[d]) x = border_map(a.border, a.ext[d], x);
+            off += x * a.ist[d];
+        }
+        total = total + load_f64((const TIn*)a.in + off) * a.tap_val[t];
+    }
+    store_from_f64((TOut*)a.out + ooff, total, NDL_ROUND_TRUNC);
+}
+
+hipError_t launch_convolve_nd(const NdArgs& a, int in_dt, int out_dt, hipStream_t s)
+{
+    if (a.total == 0) return hipSuccess;
+    uint32_t blocks = (a.total + 255u) / 256u;
+    return with_dtype(in_dt, [&](auto tin) {
+        using TIn = decltype(tin);
+        return with_dtype(out_dt, [&](auto tout) {
+            using TOut = decltype(tout);
+            hipLaunchKernelGGL((convolve_nd_kernel<TIn, TOut>), dim3(blocks), dim3(256), 0, s, a);
+            return hipGetLastError();
+        });
+    });
+}
+
+// ------------------------------------------------------ tap gradients
+// blockIdx.y = tap.  Each block walks a fixed, strided share of positions,
+// reduces in FP64 through LDS in a fixed tree, so the sum is deterministic.
+template <class TG, class TU>
+__global__ __launch_bounds__(256) void tapgrad_partial_kernel(const TapGradArgs a)
+{
+    __shared__ double red[256];
+    const int t = blockIdx.y;
+    double acc = 0;
+    const uint32_t step = 256u * (uint32_t)a.nblocks;
+    for (uint32_t idx = blockIdx.x * 256u + threadIdx.x; idx < a.total; idx += step) {
+        int64_t goff = 0, uoff = 0, j = 0;
+        uint32_t rem = idx;
+        for (int d = 0; d < a.nd; d++) {
+            uint32_t q = a.fdiv[d].div(rem);
+            int64_t c = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+            rem = q;
+            goff += c * a.gst[d];
+            uoff += c * a.ust[d];
+            if (d == a.axis_pos) j = c;
+        }
+        int64_t x = j + t - a.center;
+        if (x < 0 || x >= a.m) x = border_map(a.border, a.m, x);
+        double gv = load_f64((const TG*)a.g + goff);
+        double uv = load_f64((const TU*)a.u + (uoff - j * a.u_as + x * a.u_as));
+        acc = acc + gv * uv;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[(size_t)blockIdx.x * a.K + t] = red[0];
+}
+
+__global__ void tapgrad_final_kernel(const double* partial, int nblocks, int K, double* out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= K) return;
+    double s = 0;
+    for (int b = 0; b < nblocks; b++) s = s + partial[(size_t)b * K + t];
+    out[t] = s;
+}
+
+hipError_t launch_tapgrad_partial(const TapGradArgs& a, int g_dt, int u_is_f64, hipStream_t s)
+{
+    auto go = [&](auto tu) {
+        using TU = decltype(tu);
+        return with_dtype(g_dt, [&](auto tg) {
+            using TG = decltype(tg);
+            hipLaunchKernelGGL((tapgrad_partial_kernel<TG, TU>), dim3(a.nblocks, a.K), dim3(256), 0, s, a);
+            return hipGetLastError();
+        });
+    };
+    return u_is_f64 ? go(double{}) : go(float{});
+}
+
+hipError_t launch_tapgrad_final(const double* partial, int nblocks, int K, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(tapgrad_final_kernel, dim3((K + 63) / 64), dim3(64), 0, s, partial, nblocks, K, out);
+    return hipGetLastError();
+}
+
+} // namespace ndl

@@ -1,0 +1,106 @@
+// ndl_internal.hpp -- launch interfaces shared between the kernel files and
+// the C ABI implementation (api.hip).  Not installed; not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ndl_common.hpp"
+
+namespace ndl {
+
+// ------------------------------------------------------- generic axis pass
+// One output per thread, outputs enumerated in `order` (dims sorted by
+// |src stride|, so a wavefront reads contiguous memory for every tap).
+enum PassMode { PASS_GATHER = 0, PASS_TABLE = 1 };
+
+struct AxisPassArgs {
+    const void* in;
+    void* out;
+    int64_t m;       // extent along the convolved axis
+    int64_t in_as;   // src stride along the axis
+    int32_t nd;      // iteration dims (the axis included)
+    int32_t axis_pos; // position of the axis in the iteration order
+    int64_t ext[NDL_MAX_DIM];
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t total;  // outputs (< 2^32)
+    int32_t K, center, border, rounding, mode;
+    int32_t EL, ER;       // PASS_TABLE: edge rows with their own weights
+    const void* table;    // PASS_TABLE: (EL+ER) rows of K weights (Acc type), device
+    double taps_d[NDL_MAX_TAPS];
+    float taps_f[NDL_MAX_TAPS];
+};
+
+// accum: NDL_ACCUM_REF (FP64, separately rounded) or NDL_ACCUM_F32 (FP32 FMA)
+hipError_t launch_axis_pass(const AxisPassArgs& a, int in_dt, int out_dt, int accum, hipStream_t s);
+
+// ------------------------------------------------------ dense N-D kernel
+struct NdArgs {
+    const void* in;
+    void* out;
+    int32_t nd;
+    int64_t ext[NDL_MAX_DIM];
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t total;
+    int32_t border;
+    int32_t ntaps;
+    const int32_t* tap_off; // ntaps*nd offsets (coordinate - center), device
+    const double* tap_val;  // ntaps nonzero weights, device
+};
+hipError_t launch_convolve_nd(const NdArgs& a, int in_dt, int out_dt, hipStream_t s);
+
+// ------------------------------------------- tap-gradient partial sums
+// partial[blk*K + t] = sum over the block's share of positions p of
+// g[p] * u[B(p_axis + t - c)] (FP64), later summed in block order.
+struct TapGradArgs {
+    const void* g;      // grad_out, strided
+    const void* u;      // dense intermediate (Acc type)
+    int32_t nd, axis_pos;
+    int64_t ext[NDL_MAX_DIM];
+    int64_t gst[NDL_MAX_DIM];
+    int64_t ust[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t total;
+    int64_t m, u_as;
+    int32_t K, center, border;
+    double* partial;    // [nblocks][K]
+    int32_t nblocks;
+};
+hipError_t launch_tapgrad_partial(const TapGradArgs& a, int g_dt, int u_is_f64, hipStream_t s);
+hipError_t launch_tapgrad_final(const double* partial, int nblocks, int K, double* out, hipStream_t s);
+
+// ------------------------------------------------------ fused tile kernel
+// Up to three convolved axes in one pass over HBM: a tile of TX x TY columns
+// (x = the unit-stride dimension) marching along z with a register ring.
+struct FusedArgs {
+    const void* src;
+    void* dst;
+    int32_t X, Y, Z;
+    int64_t sy, sz, dy, dz;       // element strides (x stride is 1)
+    int32_t nb;                   // batch dims
+    int64_t bext[NDL_MAX_DIM];
+    int64_t sb[NDL_MAX_DIM], db[NDL_MAX_DIM];
+    FastDiv bdiv[NDL_MAX_DIM];
+    int32_t kx, cx, bx, ky, cy, by, kz, cz, bz;
+    int32_t tz;                   // z outputs per tile
+    int32_t tiles_x, tiles_y, tiles_z;
+    uint32_t ntiles;              // tiles_x*tiles_y*tiles_z*batch
+    FastDiv fdx, fdy, fdz;
+    int32_t xcd_remap;            // 1: give each XCD a contiguous range of tiles
+    float wx[32], wy[32], wzlag[32];   // float accumulation taps
+    int32_t iwx[32], iwy[32], iwzlag[32]; // integer accumulation taps
+};
+struct FusedConfig {
+    int tx, ty, ring, kmax;       // template selection
+};
+// Returns hipErrorInvalidValue when no instantiation matches cfg/dtype.
+hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool int_acc,
+                        hipStream_t s);
+int fused_ring_for(int kz);       // smallest ring period >= kz, or -1
+size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool acc8);
+
+} // namespace ndl

@@ -1,0 +1,309 @@
+"""GPU parity: libndl_gpu.so (through the C ABI) against the CPU oracle and the
+reference-generated fixtures.
+
+Bars: NDL_ACCUM_REF is bit-identical to the reference for every element type;
+integer images are bit-exact on every path; float32 with FP32 accumulation is
+within 1e-6 relative of the reference output (U[0,1) data, positive taps);
+for signed data the FP32 error is bounded relative to sum|w||x|.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import ndl_amd
+import oracle
+from ndl_amd import abi, workloads
+from tests.golden.make_golden import all_fixtures, load
+
+pytestmark = pytest.mark.gpu
+
+REF, F32 = abi.ACCUM_REF, abi.ACCUM_F32
+TRUNC, NEAREST = abi.ROUND_TRUNC, abi.ROUND_NEAREST
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_sep(x, axes, taps, borders, accum, rounding=TRUNC, centers=None, view=None):
+    """Run on a device copy of x (or of a view of it: `view` is an index tuple)."""
+    d = ndl_amd.DeviceArray.from_numpy(x)
+    src = d.view(view) if view is not None else d
+    out = ndl_amd.DeviceArray(src.shape, x.dtype)
+    ndl_amd.sepconv_dev(src, out, axes, taps, borders, accum, rounding, centers)
+    return out.to_numpy()
+
+
+def rel_err(got, want):
+    want = want.astype(np.float64)
+    return float(np.max(np.abs(got.astype(np.float64) - want) / np.maximum(np.abs(want), 1e-30)))
+
+
+def bound_err(got, want, x, taps):
+    """max |got-want| / (sum|w| per axis product * max|x|)"""
+    scale = float(np.max(np.abs(x))) * float(np.prod([np.sum(np.abs(t)) for t in taps]))
+    return float(np.max(np.abs(got.astype(np.float64) - want.astype(np.float64)))) / max(scale, 1e-300)
+
+
+# ------------------------------------------------------- reference fixtures
+@pytest.mark.parametrize("path", all_fixtures(), ids=lambda p: p.split("/")[-1][:-4])
+def test_golden_fixture_bit_exact(path):
+    x, out, m = load(path)
+    say("fixture", path)
+    if m["kind"] == "gaussian":
+        got = ndl_amd.gaussian_blur(x, m["sigma"], m["border"], accum=REF)
+    elif m["kind"] == "convolve":
+        got = ndl_amd.convolve(x, m["kernel"], m["border"])
+    else:
+        got = gpu_sep(x, m["axes"], m["taps"], m["borders"], REF, m["rounding"])
+    np.testing.assert_array_equal(got.view(np.uint8), out.view(np.uint8))
+
+
+# --------------------------------------------------------------- configs
+def test_config1_2d_gaussian_clamp():
+    x, axes, taps, borders = workloads.config1()
+    want = oracle.sepconv(x, axes, taps, borders)
+    say("config1 512x512 f32")
+    got = gpu_sep(x, axes, taps, borders, F32)
+    assert rel_err(got, want) <= 1e-6
+    got = gpu_sep(x, axes, taps, borders, REF)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [37, 64, 256])
+def test_config2a_3d_reflect(n):
+    x, axes, taps, borders = workloads.config2a(n=n)
+    want = oracle.sepconv(x, axes, taps, borders)
+    say("config2a", n)
+    got = gpu_sep(x, axes, taps, borders, F32)
+    err = rel_err(got, want)
+    say("  fp32 max rel err", err)
+    assert err <= 1e-6
+    if n <= 64:
+        np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, REF), want)
+
+
+@pytest.mark.parametrize("interleaved", [False, True])
+def test_config2b_u8_box_wrap_bit_exact(interleaved):
+    x, axes, taps, borders = workloads.config2b(interleaved=interleaved)
+    want = oracle.sepconv(x, axes, taps, borders)
+    say("config2b interleaved" if interleaved else "config2b planar")
+    for accum in (REF,):
+        np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, accum), want)
+    # closed form: window sum mod 256
+    xi = x.astype(np.int64)
+    sl = (0, 1) if interleaved else (1, 2)
+    s = sum(np.roll(np.roll(xi, dy, axis=sl[0]), dx, axis=sl[1]) for dy in range(-3, 4) for dx in range(-3, 4))
+    np.testing.assert_array_equal(want, (s % 256).astype(np.uint8))
+
+
+def test_config3_4d_small():
+    x, axes, taps, borders = workloads.config3(w=100, h=54, frames=9)
+    want = oracle.sepconv(x, axes, taps, borders)
+    say("config3 small")
+    got = gpu_sep(x, axes, taps, borders, F32)
+    assert bound_err(got, want, x, taps) <= 1e-6
+    np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, REF), want)
+
+
+# ------------------------------------------------------------- strided views
+VIEWS = [
+    (np.s_[::-1, :, :], "mirror z"),
+    (np.s_[:, ::-1, ::-1], "mirror x,y"),
+    (np.s_[:, ::2, :], "decimate y"),
+    (np.s_[1:-2, 3:, ::3], "roi + decimate x"),
+    (np.s_[4, :, :], "slice"),
+]
+
+
+@pytest.mark.parametrize("view,name", VIEWS, ids=[v[1] for v in VIEWS])
+def test_strided_views(view, name):
+    rng = np.random.default_rng(12)
+    base = rng.random((23, 30, 41)).astype(np.float32)
+    v = base[view]
+    nd = v.ndim
+    taps = [workloads.gaussian_taps(1.1)] * nd
+    borders = ["reflect", "clamp", "wrap"][:nd]
+    want = oracle.sepconv(v, list(range(nd)), taps, borders)
+    say("view", name)
+    got = gpu_sep(base, list(range(nd)), taps, borders, REF, view=view)
+    np.testing.assert_array_equal(got, want)
+    got = gpu_sep(base, list(range(nd)), taps, borders, F32, view=view)
+    assert rel_err(got, want) <= 1e-6
+
+
+def test_interleaved_color_per_channel():
+    # per_channel() over {3,W,H}: x has stride 3 (image/kernels.h:78-87)
+    rng = np.random.default_rng(13)
+    x = rng.integers(0, 256, size=(40, 52, 3), dtype=np.uint8)
+    g = workloads.gaussian_taps(1.3)
+    want = oracle.sepconv(x, [1, 2], [g, g], ["clamp", "clamp"], NEAREST)
+    np.testing.assert_array_equal(gpu_sep(x, [1, 2], [g, g], ["clamp", "clamp"], REF, NEAREST), want)
+
+
+# -------------------------------------------------------- fused-path sweep
+def sweep_cases():
+    rng = np.random.default_rng(14)
+    cases = []
+    for i in range(24):
+        nd = 2 + i % 2
+        shape = tuple(int(s) for s in rng.integers(1, 90, size=nd))
+        shape = tuple(max(s, 1) for s in shape)
+        axes = list(range(nd))
+        rng.shuffle(axes)
+        na = 1 + i % nd
+        axes = axes[:na]
+        taps, borders, centers = [], [], []
+        for a in axes:
+            m = shape[nd - 1 - a]
+            k = int(rng.integers(1, 26))
+            c = int(rng.integers(0, k)) if i % 3 == 0 else k // 2
+            b = ["clamp", "wrap", "reflect"][int(rng.integers(0, 3))]
+            if b != "clamp" and (c > m or k - 1 - c > m):
+                b = "clamp"
+            taps.append(rng.random(k) + 0.05)
+            borders.append(b)
+            centers.append(c)
+        cases.append((i, shape, axes, taps, borders, centers))
+    return cases
+
+
+@pytest.mark.parametrize("case", sweep_cases(), ids=lambda c: f"case{c[0]}")
+def test_fused_and_generic_sweep(case):
+    i, shape, axes, taps, borders, centers = case
+    rng = np.random.default_rng(100 + i)
+    x = rng.random(shape).astype(np.float32)
+    want = oracle.sepconv(x, axes, taps, borders, centers=centers)
+    say("sweep", i, shape, axes, [len(t) for t in taps], borders, centers)
+    got = gpu_sep(x, axes, taps, borders, F32, centers=centers)
+    assert rel_err(got, want) <= 1e-6
+    np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, REF, centers=centers), want)
+    for dt, st in [(np.float16, "f16")]:
+        xs = x.astype(dt)
+        want_s = oracle.sepconv(xs.astype(np.float64), axes, taps, borders, centers=centers)
+        got_s = gpu_sep(xs, axes, taps, borders, F32, centers=centers).astype(np.float64)
+        # output rounding (half an ulp of f16) plus FP32 accumulation
+        assert np.max(np.abs(got_s - want_s) / np.maximum(np.abs(want_s), 1e-3)) <= 2 ** -11 + 1e-5, st
+
+
+def test_zero_taps_are_skipped():
+    # detail::kernelIncludedTaps: a zero tap never multiplies (no 0*inf)
+    x = np.ones((6, 9), dtype=np.float32)
+    x[2, 4] = np.inf
+    g = np.array([-0.5, 0.0, 0.5])
+    want = oracle.sepconv(x, [0], [g], ["reflect"])
+    for accum in (REF, F32):
+        got = gpu_sep(x, [0], [g], ["reflect"], accum)
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+        np.testing.assert_array_equal(got[np.isfinite(want)], want[np.isfinite(want)])
+
+
+def test_max_reach_equals_extent():
+    # r == extent is the largest reach the single-fold maps define
+    rng = np.random.default_rng(15)
+    x = rng.random((7, 9, 10)).astype(np.float32)
+    taps = [np.ones(21), np.ones(19), np.ones(15)]
+    for b in ["wrap", "reflect"]:
+        want = oracle.sepconv(x, [0, 1, 2], taps, [b] * 3)
+        np.testing.assert_array_equal(gpu_sep(x, [0, 1, 2], taps, [b] * 3, REF), want)
+        assert rel_err(gpu_sep(x, [0, 1, 2], taps, [b] * 3, F32), want) <= 1e-6
+
+
+def test_reference_types_bit_exact():
+    rng = np.random.default_rng(16)
+    for dt in [np.uint8, np.int8, np.uint16, np.int16, np.int32, np.float64]:
+        info = np.iinfo(dt) if np.issubdtype(dt, np.integer) else None
+        x = (rng.integers(info.min, info.max, size=(11, 13, 17)).astype(dt) if info
+             else rng.standard_normal((11, 13, 17)))
+        taps = [rng.standard_normal(5), workloads.gaussian_taps(0.9), np.ones(3)]
+        for rnd in (TRUNC, NEAREST):
+            want = oracle.sepconv(x, [0, 1, 2], taps, ["reflect", "clamp", "wrap"], rnd)
+            got = gpu_sep(x, [0, 1, 2], taps, ["reflect", "clamp", "wrap"], REF, rnd)
+            np.testing.assert_array_equal(got, want, err_msg=str(dt))
+
+
+def test_convolve_nd_kernels():
+    rng = np.random.default_rng(17)
+    img = np.arange(1, 17, dtype=np.int32).reshape(4, 4)
+    for b in ["clamp", "wrap", "reflect"]:
+        for k in [np.ones((3, 3)), np.ones((5, 5)) if b == "clamp" else np.ones((3, 5))]:
+            np.testing.assert_array_equal(ndl_amd.convolve(img, k, b), oracle.convolve(img, k, b))
+    x = rng.random((9, 10, 11))
+    k = rng.standard_normal((3, 5, 2))
+    np.testing.assert_array_equal(ndl_amd.convolve(x, k, "reflect"), oracle.convolve(x, k, "reflect"))
+
+
+# ------------------------------------------------------------------ adjoint
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_adjoint_input(border, symmetric):
+    rng = np.random.default_rng(18)
+    shape = (19, 23, 29)
+    taps = ([workloads.gaussian_taps(s) for s in (0.8, 1.2, 1.6)] if symmetric
+            else [rng.standard_normal(k) for k in (5, 7, 9)])
+    g = rng.standard_normal(shape)
+    want = oracle.adjoint_input(g, [0, 1, 2], taps, [border] * 3)
+    say("adjoint", border, symmetric)
+    d = ndl_amd.DeviceArray.from_numpy(g)
+    out = ndl_amd.DeviceArray(shape, np.float64)
+    ndl_amd.adjoint_input_dev(d, out, [0, 1, 2], taps, [border] * 3, REF)
+    got = out.to_numpy()
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-11 * np.abs(want).max())
+    g32 = g.astype(np.float32)
+    d = ndl_amd.DeviceArray.from_numpy(g32)
+    out = ndl_amd.DeviceArray(shape, np.float32)
+    ndl_amd.adjoint_input_dev(d, out, [0, 1, 2], taps, [border] * 3, F32)
+    want32 = oracle.adjoint_input(g32.astype(np.float64), [0, 1, 2], taps, [border] * 3)
+    assert bound_err(out.to_numpy(), want32, g32, taps) <= 1e-6
+
+
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+def test_adjoint_taps(border):
+    rng = np.random.default_rng(19)
+    shape = (13, 17, 21)
+    taps = [rng.standard_normal(k) for k in (3, 5, 9)]
+    x = rng.standard_normal(shape)
+    w = rng.standard_normal(shape)
+    want = oracle.adjoint_taps(x, w, [0, 1, 2], taps, [border] * 3)
+    dx = ndl_amd.DeviceArray.from_numpy(x)
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    out = ndl_amd.DeviceArray((17,), np.float64)
+    ndl_amd.adjoint_taps_dev(dx, dw, [0, 1, 2], taps, [border] * 3, out, REF)
+    np.testing.assert_allclose(out.to_numpy(), want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+
+
+# ------------------------------------------------------------------ batched
+def test_plan_matches_single_calls():
+    rng = np.random.default_rng(20)
+    shapes = [(70, 90), (33, 41, 29), (5, 300), (64, 64, 64)]
+    srcs, dsts, items, keep = [], [], [], []
+    for i, shp in enumerate(shapes):
+        x = rng.random(shp).astype(np.float32)
+        nd = len(shp)
+        taps = [workloads.gaussian_taps(0.6 + 0.4 * a) for a in range(nd)]
+        b = ["clamp", "wrap", "reflect", "reflect"][i]
+        s = ndl_amd.DeviceArray.from_numpy(x)
+        d = ndl_amd.DeviceArray(shp, np.float32)
+        ks = abi.Kernels(taps, [b] * nd)
+        it = abi.BatchItem()
+        it.src, it.dst, it.naxes = s.tensor(), d.tensor(), nd
+        for a in range(nd):
+            it.axes[a] = a
+            it.k[a] = ks.arr[a]
+        srcs.append((x, taps, b))
+        dsts.append(d)
+        items.append(it)
+        keep += [s, ks]
+    arr = (abi.BatchItem * len(items))(*items)
+    import ctypes as C
+    plan = C.c_void_p()
+    ndl_amd.check(ndl_amd.lib().ndl_plan_create(arr, len(items), F32, C.byref(plan)))
+    ndl_amd.check(ndl_amd.lib().ndl_plan_execute(plan, None))
+    ndl_amd.synchronize()
+    for (x, taps, b), d in zip(srcs, dsts):
+        nd = x.ndim
+        single = gpu_sep(x, list(range(nd)), taps, [b] * nd, F32)
+        np.testing.assert_array_equal(d.to_numpy(), single)
+    ndl_amd.check(ndl_amd.lib().ndl_plan_destroy(plan))
