@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -253,7 +254,8 @@ int env_int(const char* name, int dflt)
 // Fills FusedArgs for src/dst if the fused kernel applies; returns false
 // otherwise (the caller then runs the per-axis chain).
 bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
-                const ndl_axis_kernel* k, int accum, FusedArgs& fa, FusedConfig& cfg, bool& int_acc)
+                const ndl_axis_kernel* k, int accum, FusedArgs& fa, FusedConfig& cfg, bool& int_acc,
+                int tz_override = 0)
 {
     if (env_int("NDL_DISABLE_FUSED", 0)) return false;
     if (src->dtype != dst->dtype || naxes > 3) return false;
@@ -358,7 +360,7 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
         tz = std::max(tz, std::max(16, 2 * fa.kz));
         tz = std::min(tz, fa.Z);
     }
-    int tz_env = env_int("NDL_FUSED_TZ", 0);
+    int tz_env = tz_override > 0 ? tz_override : env_int("NDL_FUSED_TZ", 0);
     if (tz_env > 0) tz = std::min(tz_env, fa.Z);
     fa.tz = tz;
     fa.tiles_z = (fa.Z + tz - 1) / tz;
@@ -768,11 +770,66 @@ NDL_API const char* ndl_last_error(void) { return g_err.c_str(); }
 NDL_API int ndl_version(void) { return 100; }
 
 // ---------------------------------------------------------------- batched
+// A plan groups its fused-eligible items by storage type into device job
+// tables; execute() runs one persistent work-queue launch per group
+// (captured once into a hipGraph and replayed), then any remaining items
+// through the per-axis chain.
+namespace {
+struct PlanGroup {
+    int dt = 0;
+    bool int_acc = false;
+    FusedConfig cfg{};
+    size_t lds = 0;
+    std::vector<FusedArgs> jobs;
+    std::vector<uint32_t> tile_start;
+    void* d_jobs = nullptr;
+    void* d_ts = nullptr;
+};
+} // namespace
+
 struct ndl_plan {
     std::vector<ndl_batch_item> items;
     std::vector<std::vector<double>> taps; // owned copies, NDL_MAX_DIM per item
     int32_t accum = NDL_ACCUM_F32;
+    std::vector<PlanGroup> groups;
+    std::vector<int> fallback;
+    uint32_t* d_counter = nullptr;
+    uint32_t grid = 1024;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool tried_graph = false;
+    ~ndl_plan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        for (auto& g : groups) {
+            if (g.d_jobs) (void)hipFree(g.d_jobs);
+            if (g.d_ts) (void)hipFree(g.d_ts);
+        }
+        if (d_counter) (void)hipFree(d_counter);
+    }
 };
+
+namespace {
+int launch_groups(ndl_plan* p, hipStream_t s)
+{
+    for (size_t i = 0; i < p->groups.size(); i++) {
+        PlanGroup& g = p->groups[i];
+        FusedQueue q;
+        q.jobs = (const FusedArgs*)g.d_jobs;
+        q.tile_start = (const uint32_t*)g.d_ts;
+        q.njobs = (int)g.jobs.size();
+        q.total = g.tile_start.back();
+        q.counter = p->d_counter + i;
+        q.grid = std::min<uint32_t>(q.total, p->grid);
+        hipError_t e = launch_fused_queue(q, g.cfg, g.dt, g.int_acc, g.lds, s);
+        if (e != hipSuccess) return hip_fail(e, "work-queue launch");
+    }
+    return NDL_OK;
+}
+} // namespace
 
 NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accum, ndl_plan** plan)
 {
@@ -780,35 +837,107 @@ NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accu
     *plan = nullptr;
     if (n < 0 || (n > 0 && !items)) return fail(NDL_ERR_ARG, "bad item list");
     if (accum != NDL_ACCUM_REF && accum != NDL_ACCUM_F32) return fail(NDL_ERR_ARG, "unknown accumulation mode %d", accum);
-    ndl_plan* p = new ndl_plan();
+    std::unique_ptr<ndl_plan> p(new ndl_plan());
     p->accum = accum;
     p->items.assign(items, items + n);
     p->taps.resize((size_t)n * NDL_MAX_DIM);
     for (int i = 0; i < n; i++) {
         ndl_batch_item& it = p->items[i];
         int rc = check_forward_args(&it.src, &it.dst, it.naxes, it.axes, it.k, accum, NDL_ROUND_TRUNC);
-        if (rc) {
-            delete p;
-            return fail(rc, "item %d: %s", i, g_err.c_str());
-        }
+        if (rc) return fail(rc, "item %d: %s", i, g_err.c_str());
         for (int a = 0; a < it.naxes; a++) {
             auto& v = p->taps[(size_t)i * NDL_MAX_DIM + a];
             v.assign(it.k[a].taps, it.k[a].taps + it.k[a].count);
             it.k[a].taps = v.data();
         }
     }
-    *plan = p;
+    // fused-eligible items, grouped by storage type; most expensive tiles first
+    struct Cand { int item; FusedArgs fa; FusedConfig cfg; bool int_acc; double cost; };
+    std::vector<Cand> cands;
+    const int tz_plan = env_int("NDL_PLAN_TZ", 64);
+    for (int i = 0; i < n; i++) {
+        ndl_batch_item& it = p->items[i];
+        if (numel(&it.src) == 0) continue;
+        Cand c;
+        c.item = i;
+        if (plan_fused(&it.src, &it.dst, it.naxes, it.axes, it.k, accum, c.fa, c.cfg, c.int_acc, tz_plan)) {
+            c.fa.xcd_remap = 0;
+            c.cost = (double)(std::min(c.fa.tz, c.fa.Z) + c.fa.kz - 1);
+            cands.push_back(c);
+        } else {
+            p->fallback.push_back(i);
+        }
+    }
+    std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.cost > b.cost; });
+    for (auto& c : cands) {
+        int dt = p->items[c.item].src.dtype;
+        PlanGroup* g = nullptr;
+        for (auto& gg : p->groups)
+            if (gg.dt == dt && gg.int_acc == c.int_acc) g = &gg;
+        if (!g) {
+            p->groups.emplace_back();
+            g = &p->groups.back();
+            g->dt = dt;
+            g->int_acc = c.int_acc;
+            g->cfg = c.cfg;
+            g->tile_start.push_back(0);
+        }
+        g->cfg.ring = std::max(g->cfg.ring, c.cfg.ring);
+        g->lds = std::max(g->lds, fused_lds_bytes(c.fa, c.cfg, false));
+        g->jobs.push_back(c.fa);
+        g->tile_start.push_back(g->tile_start.back() + c.fa.ntiles);
+    }
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    p->grid = (uint32_t)std::max(1, cus) * (uint32_t)env_int("NDL_PLAN_WG_PER_CU", 4);
+    hipError_t e = hipMalloc((void**)&p->d_counter, sizeof(uint32_t) * std::max<size_t>(1, p->groups.size()));
+    if (e != hipSuccess) return fail(NDL_ERR_NOMEM, "plan counter: %s", hipGetErrorString(e));
+    for (auto& g : p->groups) {
+        size_t jb = g.jobs.size() * sizeof(FusedArgs), tb = g.tile_start.size() * sizeof(uint32_t);
+        if ((e = hipMalloc(&g.d_jobs, jb)) != hipSuccess || (e = hipMalloc(&g.d_ts, tb)) != hipSuccess)
+            return fail(NDL_ERR_NOMEM, "plan tables: %s", hipGetErrorString(e));
+        if ((e = hipMemcpy(g.d_jobs, g.jobs.data(), jb, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(g.d_ts, g.tile_start.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, "plan table upload");
+    }
+    *plan = p.release();
     return NDL_OK;
 }
 
-NDL_API int ndl_plan_execute(ndl_plan* plan, void* stream)
+NDL_API int ndl_plan_execute(ndl_plan* p, void* stream)
 {
-    if (!plan) return fail(NDL_ERR_ARG, "null plan");
-    for (size_t i = 0; i < plan->items.size(); i++) {
-        const ndl_batch_item& it = plan->items[i];
-        if (numel(&it.src) == 0) continue;
-        int rc = forward_impl(&it.src, &it.dst, it.naxes, it.axes, it.k, plan->accum, NDL_ROUND_TRUNC,
-                              (hipStream_t)stream);
+    if (!p) return fail(NDL_ERR_ARG, "null plan");
+    hipStream_t s = (hipStream_t)stream;
+    if (!p->groups.empty()) {
+        if (!p->tried_graph && env_int("NDL_PLAN_GRAPH", 1)) {
+            // capture the work-queue launches once on a private stream
+            p->tried_graph = true;
+            if (hipStreamCreate(&p->cap) == hipSuccess &&
+                hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                int rc = launch_groups(p, p->cap);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(p->cap, &g);
+                if (rc == NDL_OK && e == hipSuccess && g &&
+                    hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) == hipSuccess) {
+                    p->graph = g;
+                } else {
+                    if (g) (void)hipGraphDestroy(g);
+                    p->exec = nullptr;
+                }
+            }
+            (void)hipGetLastError();
+        }
+        if (p->exec) {
+            hipError_t e = hipGraphLaunch(p->exec, s);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+        } else {
+            int rc = launch_groups(p, s);
+            if (rc) return rc;
+        }
+    }
+    for (int i : p->fallback) {
+        const ndl_batch_item& it = p->items[i];
+        int rc = forward_impl(&it.src, &it.dst, it.naxes, it.axes, it.k, p->accum, NDL_ROUND_TRUNC, s);
         if (rc) return rc;
     }
     return NDL_OK;

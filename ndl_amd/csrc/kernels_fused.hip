@@ -92,8 +92,9 @@ __device__ __forceinline__ void ring_dispatch(int u, Acc (&ring)[CPT][P], const 
     }
 }
 
+// One tile: `tile` indexes (x tile, y tile, z chunk, batch) with x fastest.
 template <class TS, class Acc, int TX, int TY, int P, int KMAX>
-__global__ __launch_bounds__(256) void fused_tile_kernel(const FusedArgs a)
+__device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, unsigned char* smem_raw)
 {
     constexpr int NT = 256;
     constexpr int CPT = TX * TY / NT;
@@ -106,7 +107,6 @@ __global__ __launch_bounds__(256) void fused_tile_kernel(const FusedArgs a)
     constexpr int NLMAX = (NCMAX * NRMAX + NT - 1) / NT;
     constexpr int NTASKMAX = (NCMAX * (TY / RY) + NT - 1) / NT;
 
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int NC = TX + a.kx - 1;
     const int NR = TY + a.ky - 1;
     const int NCP = (NC + 3) & ~3;
@@ -114,12 +114,6 @@ __global__ __launch_bounds__(256) void fused_tile_kernel(const FusedArgs a)
     Acc* sIn = (Acc*)smem_raw;
     Acc* sY = sIn + ((NIN + 3) & ~3);
 
-    // ---- which tile (each XCD gets a contiguous range when remapping)
-    uint32_t tile = blockIdx.x;
-    if (a.xcd_remap) {
-        const uint32_t per = a.ntiles >> 3;
-        tile = (tile & 7u) * per + (tile >> 3);
-    }
     uint32_t q = a.fdx.div(tile);
     const int tix = (int)(tile - q * (uint32_t)a.tiles_x);
     uint32_t q2 = a.fdy.div(q);
@@ -248,6 +242,50 @@ __global__ __launch_bounds__(256) void fused_tile_kernel(const FusedArgs a)
     }
 }
 
+template <class TS, class Acc, int TX, int TY, int P, int KMAX>
+__global__ __launch_bounds__(256) void fused_tile_kernel(const FusedArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // each XCD gets a contiguous range of tiles when remapping (workgroups
+    // are dealt round-robin over the 8 XCDs)
+    uint32_t tile = blockIdx.x;
+    if (a.xcd_remap) {
+        const uint32_t per = a.ntiles >> 3;
+        tile = (tile & 7u) * per + (tile >> 3);
+    }
+    fused_tile<TS, Acc, TX, TY, P, KMAX>(a, tile, smem_raw);
+}
+
+// Persistent work queue over many images: workgroups take tiles from one
+// counter until all `total` tiles of all jobs are done.  tile_start[j] is the
+// first global tile of job j (prefix sums, tile_start[njobs] = total).
+template <class TS, class Acc, int TX, int TY, int P, int KMAX>
+__global__ __launch_bounds__(256) void fused_queue_kernel(const FusedArgs* __restrict__ jobs,
+                                                          const uint32_t* __restrict__ tile_start, int njobs,
+                                                          uint32_t total, uint32_t* counter)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ uint32_t next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t t = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();
+        if (t >= total) break; // every workgroup reaches this once the counter passes total
+        int lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (tile_start[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        fused_tile<TS, Acc, TX, TY, P, KMAX>(jobs[lo], t - tile_start[lo], smem_raw);
+        __syncthreads(); // LDS is reused by the next tile
+    }
+}
+
+__global__ void reset_counter_kernel(uint32_t* c) { *c = 0u; }
+
 // ------------------------------------------------------------- dispatch
 constexpr int kRings[] = {1, 5, 9, 13, 19, 25};
 
@@ -275,7 +313,49 @@ hipError_t launch_tile(const FusedArgs& a, const FusedConfig& cfg, size_t lds, h
     return hipErrorInvalidValue;
 }
 
+template <class TS, class Acc, int TX, int TY, int KMAX>
+hipError_t launch_queue_ring(const FusedQueue& q, int ring, size_t lds, hipStream_t s)
+{
+    dim3 grid(q.grid), block(256);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, q.counter);
+    switch (ring) {
+#define NDL_QCASE(R)                                                                                         \
+    case R:                                                                                                  \
+        hipLaunchKernelGGL((fused_queue_kernel<TS, Acc, TX, TY, R, KMAX>), grid, block, lds, s, q.jobs,      \
+                           q.tile_start, q.njobs, q.total, q.counter);                                      \
+        break;
+        NDL_QCASE(1) NDL_QCASE(5) NDL_QCASE(9) NDL_QCASE(13) NDL_QCASE(19) NDL_QCASE(25)
+#undef NDL_QCASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class TS, class Acc>
+hipError_t launch_queue_tile(const FusedQueue& q, const FusedConfig& cfg, size_t lds, hipStream_t s)
+{
+    if (cfg.kmax != 25) return hipErrorInvalidValue;
+    if (cfg.tx == 64 && cfg.ty == 16) return launch_queue_ring<TS, Acc, 64, 16, 25>(q, cfg.ring, lds, s);
+    return hipErrorInvalidValue;
+}
+
 } // namespace
+
+hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int dt, bool int_acc, size_t lds,
+                              hipStream_t s)
+{
+    if (q.total == 0) return hipSuccess;
+    if (int_acc) {
+        if (dt == NDL_U8) return launch_queue_tile<uint8_t, int32_t>(q, cfg, lds, s);
+        return hipErrorInvalidValue;
+    }
+    switch (dt) {
+    case NDL_F32: return launch_queue_tile<float, float>(q, cfg, lds, s);
+    case NDL_F16: return launch_queue_tile<f16_t, float>(q, cfg, lds, s);
+    case NDL_BF16: return launch_queue_tile<bf16_t, float>(q, cfg, lds, s);
+    }
+    return hipErrorInvalidValue;
+}
 
 int fused_ring_for(int kz)
 {

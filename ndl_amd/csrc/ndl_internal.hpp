@@ -100,6 +100,17 @@ struct FusedConfig {
 // Returns hipErrorInvalidValue when no instantiation matches cfg/dtype.
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool int_acc,
                         hipStream_t s);
+// Persistent work queue over many jobs (device arrays owned by a plan).
+struct FusedQueue {
+    const FusedArgs* jobs;        // device, njobs entries
+    const uint32_t* tile_start;   // device, njobs+1 prefix sums of tile counts
+    int njobs;
+    uint32_t total;               // tiles over all jobs
+    uint32_t* counter;            // device work counter, reset by the launch
+    uint32_t grid;                // persistent workgroups
+};
+hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int dt, bool int_acc, size_t lds,
+                              hipStream_t s);
 int fused_ring_for(int kz);       // smallest ring period >= kz, or -1
 size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool acc8);
 

@@ -1,0 +1,263 @@
+// test_dropin.cpp -- the drop-in C++ API (include/ndl) on the GPU, written the
+// way the reference's own unit tests are ("label: Pass/Fail" lines,
+// unitTests/testHelpers.h:112-120).  Built by __graft_entry__.build(); run by
+// tests/test_gpu_dropin.py on the GPU.  Exit status = number of failures.
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include <ndl/convolution.h>
+#include <ndl/image.h>
+
+using namespace ndl;
+
+static int failures = 0;
+static void report(const char* label, bool ok)
+{
+    std::printf("%s: %s\n", label, ok ? "Pass" : "Fail");
+    std::fflush(stdout);
+    if (!ok) failures++;
+}
+
+// image_convolution_tests.cpp:13-67
+static void image_convolution()
+{
+    std::vector<int> data(16);
+    Image<int, 2> img(data.data(), {4, 4});
+    int i = 0;
+    for (auto it = img.begin(); it != img.end(); ++it) *it = ++i;
+    std::vector<int> idk = {0, 0, 0, 0, 1, 0, 0, 0, 0};
+    Image<int, 2> identity(idk.data(), {3, 3});
+    std::vector<int> outData(16);
+    Image<int, 2> out(outData.data(), {4, 4});
+    convolve(img, out, identity);
+    report("convolve() with identity kernel leaves image unchanged", out == img);
+    std::vector<int> sk(9, 1);
+    Image<int, 2> sumKernel(sk.data(), {3, 3});
+    convolve(img, out, sumKernel, BorderMode::Clamp);
+    report("convolve() sum kernel matches expected interior value", out(1, 1) == 1 + 2 + 3 + 5 + 6 + 7 + 9 + 10 + 11);
+    int clampCorner = out(0, 0);
+    report("convolve() clamp border matches hand-derived value",
+           clampCorner == 4 * img(0, 0) + 2 * img(1, 0) + 2 * img(0, 1) + img(1, 1));
+    convolve(img, out, sumKernel, BorderMode::Wrap);
+    report("convolve() wrap border differs from clamp border at the corner", out(0, 0) != clampCorner);
+    std::vector<int> sk5(25, 1);
+    Image<int, 2> sumKernel5(sk5.data(), {5, 5});
+    convolve(img, out, sumKernel5, BorderMode::Clamp);
+    int c5 = out(0, 0);
+    convolve(img, out, sumKernel5, BorderMode::Reflect);
+    report("convolve() reflect border differs from clamp border with a wider kernel", out(0, 0) != c5);
+}
+
+// composite_tests.cpp:105-133
+static void convolve_then_reduce()
+{
+    std::vector<int> data(20);
+    Image<int, 2> img(data.data(), {5, 4});
+    int i = 0;
+    for (auto it = img.begin(); it != img.end(); ++it) *it = ++i;
+    std::vector<int> kd = {1, 2, 1, 2, 4, 2, 1, 2, 1};
+    Image<int, 2> kernel(kd.data(), {3, 3});
+    std::vector<int> od(20);
+    Image<int, 2> out(od.data(), {5, 4});
+    convolve(img, out, kernel, BorderMode::Wrap);
+    long so = 0, si = 0, sk = 0;
+    for (int v : od) so += v;
+    for (int v : data) si += v;
+    for (int v : kd) sk += v;
+    report("convolve() with wrap border + sum() satisfies kernel.sum()*image.sum() identity", so == sk * si);
+}
+
+// type_coverage_tests.cpp:141-149
+static void float_average()
+{
+    std::vector<float> ad = {3, 5, 7, 9};
+    Image<float, 1> a(ad.data(), {4});
+    OwnedImage<double, 1> avgKernel({3}, {1.0 / 3, 1.0 / 3, 1.0 / 3});
+    std::array<float, 4> od{};
+    Image<float, 1> out(od.data(), {4});
+    convolve(a, out, avgKernel, BorderMode::Clamp);
+    report("float convolve() average equals (3+5+7)/3", out(1) == (3.0f + 5.0f + 7.0f) / 3.0f);
+}
+
+// convolution_tutorial.md:61-66,83-85,138-149
+static void tutorial()
+{
+    OwnedImage<int, 2> grid({5, 5});
+    for (int y = 0; y < 5; y++)
+        for (int x = 0; x < 5; x++) grid(x, y) = 5 * y + x + 1;
+    OwnedImage<double, 2> sumKernel({3, 3});
+    make_box_kernel(sumKernel);
+    OwnedImage<int, 2> out2({5, 5});
+    convolve(grid, out2, sumKernel);
+    const int table[5][5] = {{27, 33, 42, 51, 57}, {57, 63, 72, 81, 87}, {102, 108, 117, 126, 132},
+                             {147, 153, 162, 171, 177}, {177, 183, 192, 201, 207}};
+    bool ok = true;
+    for (int y = 0; y < 5; y++)
+        for (int x = 0; x < 5; x++) ok &= out2(x, y) == table[y][x];
+    report("tutorial sum-kernel table", ok);
+    convolve(grid, out2, sumKernel, BorderMode::Wrap);
+    bool w = out2(0, 0) == 99;
+    convolve(grid, out2, sumKernel, BorderMode::Reflect);
+    report("tutorial corners clamp 27 / wrap 99 / reflect 27", w && out2(0, 0) == 27);
+    OwnedImage<double, 2> impulse({11, 11});
+    impulse = 0.0;
+    impulse(5, 5) = 1000.0;
+    OwnedImage<double, 2> response({11, 11});
+    gaussian_blur(impulse, response, 1.0);
+    report("tutorial gaussian impulse centre 159.24",
+           std::round(response(5, 5) * 100) == 15924 && std::round(response(4, 5) * 100) == 9658 &&
+               std::round(response(4, 4) * 100) == 5858);
+}
+
+// per_channel over interleaved {3,W,H} colour (demo/convolution/convolution.cpp:75-82)
+static void per_channel_box()
+{
+    const int W = 37, H = 23;
+    OwnedImage<uint8_t, 3> src({3, W, H});
+    std::mt19937 rng(5);
+    for (auto it = src.begin(); it != src.end(); ++it) *it = (uint8_t)(rng() & 255);
+    OwnedImage<uint8_t, 3> dst({3, W, H});
+    OwnedImage<double, 2> box({7, 7});
+    make_box_kernel(box);
+    per_channel(src, dst, 0, [&](const auto& s, auto& d) { convolve(s, d, box, BorderMode::Wrap); });
+    bool ok = true;
+    for (int c = 0; c < 3; c++)
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++) {
+                int s = 0;
+                for (int dy = -3; dy <= 3; dy++)
+                    for (int dx = -3; dx <= 3; dx++) s += src(c, (x + dx + W) % W, (y + dy + H) % H);
+                ok &= dst(c, x, y) == (uint8_t)(s & 255);
+            }
+    report("per_channel 7x7 box on interleaved uint8 equals window sum mod 256", ok);
+    // the same through two separable passes on the strided channel views
+    OwnedImage<uint8_t, 3> dst2({3, W, H});
+    per_channel(src, dst2, 0, [&](const auto& s, auto& d) {
+        convolve_separable<Image<uint8_t, 2>, Image<uint8_t, 2>, 2>(
+            s, d, {std::vector<double>(7, 1.0), std::vector<double>(7, 1.0)}, {BorderMode::Wrap, BorderMode::Wrap});
+    });
+    report("separable 7-tap box passes equal the dense 7x7 box (uint8, wrap)", dst2 == dst);
+}
+
+// image_view_tests.cpp semantics: convolving a view equals convolving its copy
+static void views()
+{
+    OwnedImage<float, 3> base({21, 17, 13});
+    std::mt19937 rng(7);
+    std::uniform_real_distribution<float> u(0, 1);
+    for (auto it = base.begin(); it != base.end(); ++it) *it = u(rng);
+    auto views = {base.mirror(0), base.view({1, 2, 0}, {-2, -1, -1}, {2, -1, 1}), base.swap_axes(0, 2)};
+    bool ok = true;
+    for (const auto& v : views) {
+        OwnedImage<float, 3> copy(v);
+        OwnedImage<float, 3> a(v.extent()), b(v.extent());
+        gaussian_blur(v, a, 0.9, BorderMode::Reflect);
+        gaussian_blur(copy, b, 0.9, BorderMode::Reflect);
+        ok &= a == b;
+    }
+    report("gaussian_blur of a strided view equals that of its dense copy", ok);
+    auto sl = base.slice(2, 5);
+    OwnedImage<float, 2> a(sl.extent()), b(sl.extent());
+    OwnedImage<float, 2> copy(sl);
+    gaussian_blur(sl, a, 1.3, BorderMode::Wrap);
+    gaussian_blur(copy, b, 1.3, BorderMode::Wrap);
+    report("gaussian_blur of a slice equals that of its dense copy", a == b);
+    // writing through a strided destination view leaves other elements alone
+    OwnedImage<float, 3> whole({21, 17, 13});
+    whole = -1.0f;
+    auto dv = whole.view({0, 0, 0}, {-1, -1, -1}, {1, 2, 1});
+    OwnedImage<float, 3> srcv(dv.extent());
+    srcv = 2.0f;
+    gaussian_blur(srcv, dv, 1.0, BorderMode::Clamp);
+    bool untouched = true, written = true;
+    for (const auto& c : whole.coordinates()) {
+        if (c[1] % 2) untouched &= whole.at(c) == -1.0f;
+        else written &= std::fabs(whole.at(c) - 2.0f) < 1e-6f;
+    }
+    report("strided destination: only the view's elements are written", untouched && written);
+}
+
+// per-axis sigma and the FP32 contract (1e-6 relative on U[0,1) data)
+static void per_axis_fp32()
+{
+    OwnedImage<float, 3> x({40, 33, 29});
+    std::mt19937 rng(9);
+    std::uniform_real_distribution<float> u(0, 1);
+    for (auto it = x.begin(); it != x.end(); ++it) *it = u(rng);
+    OwnedImage<float, 3> ref(x.extent()), f32(x.extent());
+    std::array<double, 3> sig = {1.5, 1.5, 3.0};
+    std::array<BorderMode, 3> br = {BorderMode::Reflect, BorderMode::Reflect, BorderMode::Reflect};
+    gaussian_blur<OwnedImage<float, 3>, OwnedImage<float, 3>, 3>(x, ref, sig, br, Accumulation::Reference);
+    gaussian_blur<OwnedImage<float, 3>, OwnedImage<float, 3>, 3>(x, f32, sig, br, Accumulation::FP32);
+    double worst = 0;
+    for (const auto& c : x.coordinates()) worst = std::max(worst, std::fabs((double)f32.at(c) - ref.at(c)) / ref.at(c));
+    report("per-axis Gaussian: FP32 accumulation within 1e-6 of the reference arithmetic", worst <= 1e-6);
+    DeviceImage<float, 3> dx(x), dy(x.extent());
+    gaussian_blur<DeviceImage<float, 3>, DeviceImage<float, 3>, 3>(dx, dy, sig, br, Accumulation::FP32);
+    report("DeviceImage path equals the staged host path", dy.to_host() == f32);
+}
+
+// adjoints: <A x, w> == <x, A^T w>, and tap gradients vs central differences
+static void adjoints()
+{
+    const std::array<int, 3> e = {19, 14, 11};
+    OwnedImage<double, 3> x(e), w(e), ax(e), atw(e);
+    std::mt19937 rng(11);
+    std::normal_distribution<double> n(0, 1);
+    for (auto it = x.begin(); it != x.end(); ++it) *it = n(rng);
+    for (auto it = w.begin(); it != w.end(); ++it) *it = n(rng);
+    std::array<std::vector<double>, 3> taps;
+    for (int a = 0; a < 3; a++)
+        for (int t = 0; t < 5 + 2 * a; t++) taps[a].push_back(n(rng));
+    for (BorderMode b : {BorderMode::Clamp, BorderMode::Wrap, BorderMode::Reflect}) {
+        std::array<BorderMode, 3> br = {b, b, b};
+        convolve_separable<OwnedImage<double, 3>, OwnedImage<double, 3>, 3>(x, ax, taps, br);
+        convolve_adjoint<OwnedImage<double, 3>, OwnedImage<double, 3>, 3>(w, atw, taps, br);
+        double lhs = 0, rhs = 0;
+        for (const auto& c : x.coordinates()) {
+            lhs += ax.at(c) * w.at(c);
+            rhs += x.at(c) * atw.at(c);
+        }
+        report("adjoint dot-product identity <Ax,w> == <x,A^T w>", std::fabs(lhs - rhs) <= 1e-10 * std::fabs(lhs));
+        auto g = convolve_adjoint_taps<OwnedImage<double, 3>, OwnedImage<double, 3>, 3>(x, w, taps, br);
+        bool ok = true;
+        for (int a = 0; a < 3; a++)
+            for (std::size_t t = 0; t < taps[a].size(); t++) {
+                auto tp = taps, tm = taps;
+                tp[a][t] += 1e-3;
+                tm[a][t] -= 1e-3;
+                OwnedImage<double, 3> yp(e), ym(e);
+                convolve_separable<OwnedImage<double, 3>, OwnedImage<double, 3>, 3>(x, yp, tp, br);
+                convolve_separable<OwnedImage<double, 3>, OwnedImage<double, 3>, 3>(x, ym, tm, br);
+                double lp = 0, lm = 0;
+                for (const auto& c : x.coordinates()) {
+                    lp += w.at(c) * yp.at(c);
+                    lm += w.at(c) * ym.at(c);
+                }
+                double fd = (lp - lm) / 2e-3;
+                ok &= std::fabs(fd - g[a][t]) <= 1e-7 * std::max(1.0, std::fabs(fd));
+            }
+        report("tap gradients equal FP64 central differences", ok);
+    }
+}
+
+int main()
+{
+    int n = 0;
+    if (ndl_device_count(&n) != 0 || n < 1) {
+        std::printf("no GPU visible\n");
+        return 99;
+    }
+    image_convolution();
+    convolve_then_reduce();
+    float_average();
+    tutorial();
+    per_channel_box();
+    views();
+    per_axis_fp32();
+    adjoints();
+    std::printf("%d failures\n", failures);
+    return failures;
+}
