@@ -227,5 +227,171 @@ def cpu_baseline(x, axes, taps, borders, nvox):
     }
 
 
+# --------------------------------------------------------------------- suite
+def _time_launches(hip, stream, fn, reps, flush=None):
+    """mean ms of fn() over reps, each bracketed by events (flush outside)."""
+    import ndl_amd
+    ts = []
+    for _ in range(reps):
+        if flush:
+            flush()
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        fn()
+        hip.record(e1, stream)
+        ts.append((e0, e1))
+    ndl_amd.synchronize(stream)
+    return float(np.mean([hip.elapsed_ms(a, b) for a, b in ts]))
+
+
+def suite(reps=10, configs=(1, 2, 3, 4, 5)):
+    """Secondary configurations; one JSON line each on stdout."""
+    import ndl_amd
+    import oracle
+    from ndl_amd import abi, workloads
+
+    hip = ndl_amd.hip()
+    stream = hip.stream()
+    lib = ndl_amd.lib()
+    bu = C.CDLL(os.path.join(ROOT, "ndl_amd", "libndl_benchutil.so"))
+    bu.ndl_bench_flush.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+    dflush = ndl_amd.DeviceArray((1 << 30,), np.uint8)
+    dsink = ndl_amd.DeviceArray((4096,), np.uint32)
+
+    def flush():
+        bu.ndl_bench_flush(dflush.ptr, 1 << 30, dsink.ptr, stream)
+
+    def fwd_fn(src, dst, axes, taps, borders, accum):
+        ks = abi.Kernels(taps, borders)
+        ax = abi.Kernels.axes(axes)
+        ts, td = src.tensor(), dst.tensor()
+
+        def f():
+            ndl_amd.check(lib.ndl_sepconv_forward(C.byref(ts), C.byref(td), len(axes), ax, ks.arr, accum,
+                                                  abi.ROUND_TRUNC, stream))
+        f.keep = (ks, ax, ts, td)
+        return f
+
+    def emit(d):
+        print(json.dumps(d), flush=True)
+
+    if 1 in configs:
+        x, axes, taps, borders = workloads.config1()
+        s = ndl_amd.DeviceArray.from_numpy(x)
+        d = ndl_amd.DeviceArray(x.shape, np.float32)
+        f = fwd_fn(s, d, axes, taps, borders, abi.ACCUM_F32)
+        f()
+        ms = _time_launches(hip, stream, f, reps)
+        emit({"config": "1: 2-D f32 512x512 sigma 2 clamp, forward", "us": round(ms * 1e3, 2),
+              "gvox_s": round(x.size / ms / 1e6, 2), "GBps_alg": round(8 * x.size / ms / 1e6, 1)})
+    if 2 in configs:
+        for inter in (False, True):
+            x, axes, taps, borders = workloads.config2b(interleaved=inter)
+            s = ndl_amd.DeviceArray.from_numpy(x)
+            d = ndl_amd.DeviceArray(x.shape, np.uint8)
+            f = fwd_fn(s, d, axes, taps, borders, abi.ACCUM_REF)
+            f()
+            ms = _time_launches(hip, stream, f, reps)
+            ok = np.array_equal(d.to_numpy(), oracle.sepconv(x, axes, taps, borders))
+            emit({"config": f"2b: u8 2048x2048x3 box7 wrap ({'interleaved' if inter else 'planar'}), forward",
+                  "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
+                  "GBps_alg": round(2 * x.size / ms / 1e6, 1), "bit_exact": bool(ok)})
+    if 3 in configs:
+        x, axes, taps, borders = workloads.config3()
+        s = ndl_amd.DeviceArray.from_numpy(x)
+        d = ndl_amd.DeviceArray(x.shape, np.float32)
+        f = fwd_fn(s, d, axes, taps, borders, abi.ACCUM_F32)
+        f()
+        ms = _time_launches(hip, stream, f, max(3, reps // 2))
+        emit({"config": "3: 4-D f32 1920x1080x3x32, x,y sigma 1 clamp + t binomial wrap, forward",
+              "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
+              "GBps_alg": round(8 * x.size / ms / 1e6, 1)})
+        del s, d
+    if 4 in configs:
+        items = workloads.config4()
+        srcs, dsts, bitems, keep = [], [], [], []
+        nvox = 0
+        for i, (shape, taps, border) in enumerate(items):
+            x = workloads.config4_values(5, i, shape)
+            nvox += x.size
+            sd = ndl_amd.DeviceArray.from_numpy(x)
+            dd = ndl_amd.DeviceArray(shape, np.float32)
+            nd = len(shape)
+            ks = abi.Kernels(taps, [border] * nd)
+            it = abi.BatchItem()
+            it.src, it.dst, it.naxes = sd.tensor(), dd.tensor(), nd
+            for a in range(nd):
+                it.axes[a] = a
+                it.k[a] = ks.arr[a]
+            srcs.append(sd)
+            dsts.append(dd)
+            bitems.append(it)
+            keep.append(ks)
+        arr = (abi.BatchItem * len(bitems))(*bitems)
+        plan = C.c_void_p()
+        ndl_amd.check(lib.ndl_plan_create(arr, len(bitems), abi.ACCUM_F32, C.byref(plan)))
+        run_plan = lambda: ndl_amd.check(lib.ndl_plan_execute(plan, stream))  # noqa: E731
+        run_plan()
+        ms_plan = _time_launches(hip, stream, run_plan, max(3, reps // 2))
+        singles = [fwd_fn(srcs[i], dsts[i], list(range(len(items[i][0]))), items[i][1],
+                          [items[i][2]] * len(items[i][0]), abi.ACCUM_F32) for i in range(len(items))]
+
+        def run_singles():
+            for f in singles:
+                f()
+        run_singles()
+        ms_single = _time_launches(hip, stream, run_singles, max(3, reps // 2))
+        ndl_amd.check(lib.ndl_plan_destroy(plan))
+        emit({"config": "4: 256 uneven f32 instances (192 2-D, 64 3-D), one work-queue launch (hipGraph)",
+              "voxels": nvox, "plan_us": round(ms_plan * 1e3, 1), "plan_gvox_s": round(nvox / ms_plan / 1e6, 2),
+              "single_launches_us": round(ms_single * 1e3, 1),
+              "speedup_vs_single_launches": round(ms_single / ms_plan, 3),
+              "GBps_alg_plan": round(8 * nvox / ms_plan / 1e6, 1)})
+        del srcs, dsts
+    if 5 in configs:
+        n = 384
+        for storage in ("bf16", "f16"):
+            xs, xd, w, taps = workloads.config5(n=n, storage=storage)
+            for border in ("clamp", "reflect"):
+                axes = [0, 1, 2]
+                borders = [border] * 3
+                dx = ndl_amd.DeviceArray.from_numpy(xs, bf16=(storage == "bf16"))
+                dy = ndl_amd.DeviceArray(xs.shape, xs.dtype, bf16=(storage == "bf16"))
+                dw = ndl_amd.DeviceArray.from_numpy(w)
+                dgx = ndl_amd.DeviceArray(xs.shape, np.float32)
+                d32 = ndl_amd.DeviceArray.from_numpy(xd.astype(np.float32))
+                y32 = ndl_amd.DeviceArray(xs.shape, np.float32)
+                f_half = fwd_fn(dx, dy, axes, taps, borders, abi.ACCUM_F32)
+                f_32 = fwd_fn(d32, y32, axes, taps, borders, abi.ACCUM_F32)
+
+                def adj():
+                    ndl_amd.adjoint_input_dev(dw, dgx, axes, taps, borders, abi.ACCUM_F32, stream=stream)
+                f_half()
+                f_32()
+                adj()
+                ms_h = _time_launches(hip, stream, f_half, reps, flush)
+                ms_32 = _time_launches(hip, stream, f_32, reps, flush)
+                ms_adj = _time_launches(hip, stream, adj, reps, flush)
+                # errors vs the FP64 oracle fed the storage-rounded input
+                want = oracle.sepconv(xd, axes, taps, borders, dst=np.zeros(xd.shape))
+                got = dy.to_numpy()
+                got = (got.astype(np.uint32) << 16).view(np.float32).astype(np.float64) if storage == "bf16" \
+                    else got.astype(np.float64)
+                err = got - want
+                gxw = oracle.adjoint_input(w.astype(np.float64), axes, taps, borders)
+                egx = dgx.to_numpy().astype(np.float64) - gxw
+                emit({"config": f"5: {storage} 384^3, 3x9 random taps, {border}",
+                      "fwd_us": round(ms_h * 1e3, 2), "fwd_f32_form_us": round(ms_32 * 1e3, 2),
+                      "speedup_vs_f32": round(ms_32 / ms_h, 3), "adj_input_f32_us": round(ms_adj * 1e3, 2),
+                      "fwd_max_abs_err": float(np.max(np.abs(err))), "fwd_rms_err": float(np.sqrt(np.mean(err ** 2))),
+                      "fwd_rms_ref": float(np.sqrt(np.mean(want ** 2))),
+                      "adj_max_abs_err": float(np.max(np.abs(egx))), "adj_rms_err": float(np.sqrt(np.mean(egx ** 2)))})
+                del dx, dy, dw, dgx, d32, y32
+
+
 if __name__ == "__main__":
-    main()
+    if "--suite" in sys.argv:
+        which = [int(a) for a in sys.argv[sys.argv.index("--suite") + 1:] if a.isdigit()] or [1, 2, 3, 4, 5]
+        suite(configs=tuple(which))
+    else:
+        main()

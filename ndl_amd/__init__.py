@@ -93,7 +93,10 @@ class DeviceArray:
         self.shape = tuple(int(s) for s in shape)
         self.np_dtype = np.dtype(dtype)
         self.bf16 = bf16
-        self.ndl_dtype = abi.BF16 if bf16 else abi.dtype_of(np.empty(0, self.np_dtype))
+        try:
+            self.ndl_dtype = abi.BF16 if bf16 else abi.dtype_of(np.empty(0, self.np_dtype))
+        except TypeError:
+            self.ndl_dtype = -1  # raw buffer (e.g. uint32 scratch); not usable as a tensor
         if strides is None:
             st, acc = [], 1
             for s in reversed(self.shape):

@@ -8,7 +8,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -251,6 +254,35 @@ int env_int(const char* name, int dflt)
     return v && *v ? atoi(v) : dflt;
 }
 
+int device_cus()
+{
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cached = n;
+        else
+            cached = 256;
+    }
+    return cached;
+}
+
+// resident workgroups per CU of one fused instantiation, cached
+int occupancy(const FusedConfig& cfg, int dt, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, size_t>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple(dt, cfg.cpt, cfg.ring, lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int b = fused_blocks_per_cu(cfg, dt, lds);
+    (void)hipGetLastError();
+    cache[key] = b;
+    return b;
+}
+
 // Fills FusedArgs for src/dst if the fused kernel applies; returns false
 // otherwise (the caller then runs the per-axis chain).
 bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
@@ -263,15 +295,21 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     int_acc = false;
     if (is_f32_family(dt)) {
         if (accum != NDL_ACCUM_F32) return false;
+        // a zero tap must not multiply (detail.h:164-174: 0*inf would be NaN);
+        // the per-axis chain skips such taps
+        for (int i = 0; i < naxes; i++)
+            for (int t = 0; t < k[i].count; t++)
+                if (k[i].taps[t] == 0.0) return false;
     } else if (dt == NDL_U8) {
-        // exact integer arithmetic: every partial sum must fit in int32
-        double bound = dt == NDL_U8 ? 255.0 : 65535.0;
+        // integer taps on uint8: the kernel accumulates in float, exact while
+        // every partial sum stays below 2^24
+        double bound = 255.0;
         for (int i = 0; i < naxes; i++) {
             double as;
             if (!taps_integral(k[i], as)) return false;
             bound *= as;
         }
-        if (bound >= 2147483647.0) return false;
+        if (bound >= 16777216.0) return false;
         int_acc = true;
     } else {
         return false;
@@ -346,19 +384,26 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     cfg.kmax = 25;
     cfg.tx = 64;
     cfg.ty = 16;
+    cfg.cpt = env_int("NDL_FUSED_CPT", 2);
+    if (cfg.cpt != 1 && cfg.cpt != 2) cfg.cpt = 2;
     cfg.ring = fused_ring_for(fa.kz);
     if (cfg.ring < 0) return false;
     fa.tiles_x = (fa.X + cfg.tx - 1) / cfg.tx;
     fa.tiles_y = (fa.Y + cfg.ty - 1) / cfg.ty;
     int64_t xy_tiles = (int64_t)fa.tiles_x * fa.tiles_y * batch;
-    // z chunking: enough workgroups to cover the chip, long marches otherwise
-    int target = env_int("NDL_FUSED_TARGET_WG", 512);
+    // z chunking: tiles fill the GPU in whole rounds of resident workgroups;
+    // a chunk of tz outputs costs tz + kz - 1 slice steps
     int tz = fa.Z;
-    if (fa.Z > 1 && xy_tiles < target) {
-        int64_t chunks = (target + xy_tiles - 1) / xy_tiles;
-        tz = (int)((fa.Z + chunks - 1) / chunks);
-        tz = std::max(tz, std::max(16, 2 * fa.kz));
-        tz = std::min(tz, fa.Z);
+    if (fa.Z > 1) {
+        int64_t slots = (int64_t)device_cus() * std::max(1, occupancy(cfg, dt, fused_lds_bytes(fa, cfg, false)));
+        double best = 1e300;
+        for (int chunks = 1; chunks <= std::min(fa.Z, 256); chunks++) {
+            int t = (fa.Z + chunks - 1) / chunks;
+            int64_t blocks = xy_tiles * ((fa.Z + t - 1) / t);
+            int64_t rounds = (blocks + slots - 1) / slots;
+            double cost = (double)rounds * (t + fa.kz - 1);
+            if (cost < best - 1e-9) { best = cost; tz = t; }
+        }
     }
     int tz_env = tz_override > 0 ? tz_override : env_int("NDL_FUSED_TZ", 0);
     if (tz_env > 0) tz = std::min(tz_env, fa.Z);
@@ -770,8 +815,9 @@ NDL_API const char* ndl_last_error(void) { return g_err.c_str(); }
 NDL_API int ndl_version(void) { return 100; }
 
 // ---------------------------------------------------------------- batched
-// A plan groups its fused-eligible items by storage type into device job
-// tables; execute() runs one persistent work-queue launch per group
+// A plan groups its fused-eligible items by storage type and z tap count
+// (the register ring's compile-time period) into device job tables;
+// execute() runs one persistent work-queue launch per group
 // (captured once into a hipGraph and replayed), then any remaining items
 // through the per-axis chain.
 namespace {
@@ -873,7 +919,7 @@ NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accu
         int dt = p->items[c.item].src.dtype;
         PlanGroup* g = nullptr;
         for (auto& gg : p->groups)
-            if (gg.dt == dt && gg.int_acc == c.int_acc) g = &gg;
+            if (gg.dt == dt && gg.int_acc == c.int_acc && gg.cfg.ring == c.cfg.ring) g = &gg;
         if (!g) {
             p->groups.emplace_back();
             g = &p->groups.back();
@@ -882,7 +928,6 @@ NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accu
             g->cfg = c.cfg;
             g->tile_start.push_back(0);
         }
-        g->cfg.ring = std::max(g->cfg.ring, c.cfg.ring);
         g->lds = std::max(g->lds, fused_lds_bytes(c.fa, c.cfg, false));
         g->jobs.push_back(c.fa);
         g->tile_start.push_back(g->tile_start.back() + c.fa.ntiles);

@@ -96,6 +96,7 @@ struct FusedArgs {
 };
 struct FusedConfig {
     int tx, ty, ring, kmax;       // template selection
+    int cpt;                      // x outputs per thread (threads = tx*ty/cpt)
 };
 // Returns hipErrorInvalidValue when no instantiation matches cfg/dtype.
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool int_acc,
@@ -111,7 +112,9 @@ struct FusedQueue {
 };
 hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int dt, bool int_acc, size_t lds,
                               hipStream_t s);
-int fused_ring_for(int kz);       // smallest ring period >= kz, or -1
+int fused_ring_for(int kz);       // ring period for kz, or -1 when unsupported
+// resident workgroups per CU for this instantiation (0 if unknown)
+int fused_blocks_per_cu(const FusedConfig& cfg, int dt, size_t lds);
 size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool acc8);
 
 } // namespace ndl
