@@ -251,6 +251,28 @@ void convolve_separable(const SrcImageT& src, DstImageT& dst, const std::array<s
     detail::run_sepconv(src, dst, plan, acc, NDL_ROUND_TRUNC);
 }
 
+/// Spatial gradient (processing/convolution.h:359-388): dst.slice(0, axis) is
+/// the central difference {-0.5, 0, +0.5} along `axis` (the zero tap is
+/// skipped, as in convolve()), for every axis; dst has a leading component
+/// axis of size DIM.
+template <class SrcImageT, class DstImageT>
+void gradient(const SrcImageT& src, DstImageT& dst, BorderMode border = BorderMode::Reflect)
+{
+    using SrcT = typename SrcImageT::value_type;
+    static_assert(std::is_arithmetic_v<SrcT>, "ndl::gradient() requires an arithmetic value_type");
+    constexpr int DIM = detail::dim_of<SrcImageT>;
+    constexpr int DstDIM = detail::dim_of<DstImageT>;
+    static_assert(DstDIM == DIM + 1, "ndl::gradient() needs a destination with a leading component axis");
+    assert(dst.extent()[0] == DIM);
+    for (int d = 0; d < DIM; d++) assert(dst.extent()[d + 1] == src.extent()[d]);
+    for (int axis = 0; axis < DIM; axis++) {
+        auto component = dst.slice(0, axis);
+        detail::AxisPlan<DIM> plan;
+        plan.add(axis, {-0.5, 0.0, 0.5}, 1, border);
+        detail::run_sepconv(src, component, plan, Accumulation::Reference, NDL_ROUND_TRUNC);
+    }
+}
+
 /// Input gradient of L = <w, convolve_separable(x)>: grad_in = A^T grad_out,
 /// the exact transpose including border folding (no reference counterpart).
 template <class GradOutT, class GradInT, int DIM>

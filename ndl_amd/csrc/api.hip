@@ -269,15 +269,15 @@ int device_cus()
 }
 
 // resident workgroups per CU of one fused instantiation, cached
-int occupancy(const FusedConfig& cfg, int dt, size_t lds)
+int occupancy(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, size_t>, int> cache;
+    static std::map<std::tuple<int, int, int, int, int, size_t>, int> cache;
     std::lock_guard<std::mutex> lock(mu);
-    auto key = std::make_tuple(dt, cfg.cpt, cfg.ring, lds);
+    auto key = std::make_tuple(dt, cfg.cpt, cfg.ring, kx, ky, lds);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    int b = fused_blocks_per_cu(cfg, dt, lds);
+    int b = fused_blocks_per_cu(cfg, dt, kx, ky, lds);
     (void)hipGetLastError();
     cache[key] = b;
     return b;
@@ -329,6 +329,9 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
         if (src->stride[d] == 1 && dst->stride[d] == 1 && src->extent[d] > 1) { xd = d; break; }
     if (xd < 0) return false;
     role[xd] = 0;
+    // y = the convolved non-x dimension with the smaller stride, z = the
+    // other: every pass order gives the same tile (FP32 accumulation and the
+    // exact integer path do not depend on pass order)
     int yd = -1, zd = -1;
     for (int i = 0; i < naxes; i++) {
         int d = axes[i];
@@ -337,6 +340,7 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
         else if (zd < 0) zd = d;
         else return false;
     }
+    if (zd >= 0 && std::llabs(src->stride[zd]) < std::llabs(src->stride[yd])) std::swap(yd, zd);
     if (yd >= 0) role[yd] = 1;
     if (zd >= 0) role[zd] = 2;
     memset(&fa, 0, sizeof fa);
@@ -385,9 +389,13 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     cfg.tx = 64;
     cfg.ty = 16;
     cfg.cpt = env_int("NDL_FUSED_CPT", 2);
-    if (cfg.cpt != 1 && cfg.cpt != 2) cfg.cpt = 2;
+    if (cfg.cpt != 2 && cfg.cpt != 4) cfg.cpt = 2;
     cfg.ring = fused_ring_for(fa.kz);
     if (cfg.ring < 0) return false;
+    if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) {
+        cfg.cpt = 2;
+        if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) return false;
+    }
     fa.tiles_x = (fa.X + cfg.tx - 1) / cfg.tx;
     fa.tiles_y = (fa.Y + cfg.ty - 1) / cfg.ty;
     int64_t xy_tiles = (int64_t)fa.tiles_x * fa.tiles_y * batch;
@@ -395,7 +403,8 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     // a chunk of tz outputs costs tz + kz - 1 slice steps
     int tz = fa.Z;
     if (fa.Z > 1) {
-        int64_t slots = (int64_t)device_cus() * std::max(1, occupancy(cfg, dt, fused_lds_bytes(fa, cfg, false)));
+        int64_t slots = (int64_t)device_cus() *
+                        std::max(1, occupancy(cfg, dt, fa.kx, fa.ky, fused_lds_bytes(fa, cfg, false)));
         double best = 1e300;
         for (int chunks = 1; chunks <= std::min(fa.Z, 256); chunks++) {
             int t = (fa.Z + chunks - 1) / chunks;

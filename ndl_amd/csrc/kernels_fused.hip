@@ -1,23 +1,31 @@
 // kernels_fused.hip -- up to three convolved axes in one pass over HBM.
 //
-// A workgroup of 256 threads owns a TX x TY tile of output columns
+// A workgroup of TX*TY/CPT threads owns a TX x TY tile of output columns
 // (x = the unit-stride dimension, y = a second axis) and marches along a third
 // axis z, one input slice at a time:
 //   1. the slice's (TY+ky-1) x (TX+kx-1) input window, border indices
-//      resolved once per workgroup (detail.h:178-194 semantics), is prefetched
-//      into registers one slice ahead and published to LDS;
-//   2. y-pass: LDS -> LDS, register-blocked RY rows per task;
-//   3. x-pass: LDS -> registers, each thread CPT consecutive x outputs;
-//   4. z-pass: each thread keeps the last P x-filtered slices of its columns
-//      in a register ring; once kz slices are in, one output slice is written.
+//      resolved once per workgroup (detail.h:178-194 semantics), is requested
+//      two slices ahead into registers (two buffers) and published to LDS;
+//   2. y-pass: LDS -> LDS, 4 rows per task;
+//   3. x-pass: LDS -> registers, CPT consecutive x outputs per thread;
+//   4. z-pass: each thread keeps the last P = kz x-filtered slices of its
+//      columns in a register ring; once kz slices are in, one output slice is
+//      written.  The slice loop is unrolled by 2P so every ring slot and
+//      prefetch buffer is a compile-time register choice.
 // Every input element is read from HBM once per tile and every output written
 // once; only the y/x halos are re-read (from L2) and re-computed.
 //
-// Accumulation: FP32 FMA for float storage (f32/f16/bf16), exact int32 for
-// integer storage with integer taps (bit-identical to the reference's FP64
-// sum, which is exact for such data, convolution.h:64-70).  Exactly-zero taps
-// are skipped like detail::kernelIncludedTaps (detail.h:164-174).
+// Tap counts are compile-time for the common filters (everything unrolled,
+// taps in scalar registers, LDS offsets as immediates); a generic
+// instantiation per ring period takes kx/ky at run time (4-tap chunks).
+//
+// Accumulation: FP32 FMA.  Integer (uint8) storage reaches this kernel only
+// with integer taps and sums below 2^24, so every float value is an exact
+// integer and the result is bit-identical to the reference's FP64 sum
+// (convolution.h:64-70).  The host routes filters with an exactly-zero tap
+// to the per-axis chain, which skips them (detail.h:164-174).
 #include <type_traits>
+#include <utility>
 
 #include "ndl_internal.hpp"
 
@@ -29,12 +37,23 @@ template <class TS> __device__ __forceinline__ float load_s(const TS* p)
     if constexpr (is_int_storage<TS>::value) return (float)*p;
     else return load_f32(p);
 }
-// Integer storage only reaches this kernel with integer taps and sums below
-// 2^24, so the float value is an exact integer (convolution.h:69 low bits).
 template <class TS> __device__ __forceinline__ void store_s(TS* p, float v)
 {
-    if constexpr (is_int_storage<TS>::value) *p = (TS)(uint32_t)(int32_t)v;
+    if constexpr (is_int_storage<TS>::value) *p = (TS)(uint32_t)(int32_t)v; // exact integer, low bits (convolution.h:69)
     else store_from_f32(p, v);
+}
+
+template <int V> using IC = std::integral_constant<int, V>;
+
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(IC<I>{}), ...);
+}
+// f(IC<0>{}), ..., f(IC<N-1>{}) in order
+template <int N, class F> __device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 // acc[o] += sum_{j<NTAP} w[j] * win[o + j], o < NO, taps ascending
@@ -50,27 +69,36 @@ __device__ __forceinline__ void fma_block(float (&acc)[NO], const float (&win)[N
     }
 }
 
-// 1-D correlation of NO consecutive outputs read through `get(i)` (window
-// element i), K taps at `w`: 4-tap chunks, then a 1..3-tap tail.
-template <int NO, class Get>
+// 1-D correlation of NO consecutive outputs read through get(i) (window
+// element i).  KS > 0: KS taps, fully unrolled.  KS == 0: K taps at run time,
+// 4-tap chunks then a 1..3-tap tail.
+template <int NO, int KS, class Get>
 __device__ __forceinline__ void correlate(float (&acc)[NO], int K, const float* w, Get get)
 {
-    constexpr int NW = NO + 3;
-    int t0 = 0;
-    for (; t0 + 4 <= K; t0 += 4) {
+    if constexpr (KS > 0) {
+        constexpr int NW = NO + KS - 1;
         float win[NW];
 #pragma unroll
-        for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
-        fma_block<NO, 4, NW>(acc, win, w + t0);
-    }
-    const int rem = K - t0;
-    if (rem > 0) {
-        float win[NW];
+        for (int i = 0; i < NW; i++) win[i] = get(i);
+        fma_block<NO, KS, NW>(acc, win, w);
+    } else {
+        constexpr int NW = NO + 3;
+        int t0 = 0;
+        for (; t0 + 4 <= K; t0 += 4) {
+            float win[NW];
 #pragma unroll
-        for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
-        if (rem == 3) fma_block<NO, 3, NW>(acc, win, w + t0);
-        else if (rem == 2) fma_block<NO, 2, NW>(acc, win, w + t0);
-        else fma_block<NO, 1, NW>(acc, win, w + t0);
+            for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
+            fma_block<NO, 4, NW>(acc, win, w + t0);
+        }
+        const int rem = K - t0;
+        if (rem > 0) {
+            float win[NW];
+#pragma unroll
+            for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
+            if (rem == 3) fma_block<NO, 3, NW>(acc, win, w + t0);
+            else if (rem == 2) fma_block<NO, 2, NW>(acc, win, w + t0);
+            else fma_block<NO, 1, NW>(acc, win, w + t0);
+        }
     }
 }
 
@@ -100,39 +128,29 @@ __device__ __forceinline__ void ring_step(float (&ring)[CPT][P], const float (&x
     }
 }
 
-template <int U, int P, int CPT, class TS>
-__device__ __forceinline__ void ring_dispatch(int u, float (&ring)[CPT][P], const float (&xv)[CPT], int s,
-                                              const FusedArgs& a, TS* out, bool row_ok, int x_first)
-{
-    if constexpr (U < P) {
-        if (u == U) {
-            ring_step<U, P, CPT, TS>(ring, xv, s, a, out, row_ok, x_first);
-            return;
-        }
-        ring_dispatch<U + 1, P, CPT, TS>(u, ring, xv, s, a, out, row_ok, x_first);
-    }
-}
-
 // One tile: `tile` indexes (x tile, y tile, z chunk, batch) with x fastest.
+// KXS/KYS: compile-time x/y tap counts, 0 = taken from the arguments.
 // LDS (floats): sIn = (NR+3) x NC input window (+3 rows of slack for the
-// 4-tap chunked y reads), sY = TY x NCP y-filtered rows (NCP >= NC+3, a
-// multiple of 4, slack for the chunked x reads).
-template <class TS, int TX, int TY, int CPT, int P>
+// chunked y reads), sY = TY x NCP y-filtered rows (NCP >= NC+3, a multiple
+// of 4, slack for the chunked x reads).
+template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P>
 __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, unsigned char* smem_raw)
 {
-    constexpr int kNT = TX * TY / CPT; // threads per workgroup
+    constexpr int NT = TX * TY / CPT;
     constexpr int XG = TX / CPT;
-    static_assert(CPT >= 1 && XG * CPT == TX && XG * TY == kNT, "tile must map CPT x-outputs per thread");
+    static_assert(CPT >= 1 && XG * CPT == TX && XG * TY == NT, "tile must map CPT x-outputs per thread");
     constexpr int RY = 4;
     static_assert(TY % RY == 0, "TY must be a multiple of 4");
     constexpr int KMAX = 25;
-    constexpr int NCMAX = TX + KMAX - 1;
-    constexpr int NRMAX = TY + KMAX - 1;
-    constexpr int NLMAX = (NCMAX * NRMAX + kNT - 1) / kNT;
-    constexpr int NTASKMAX = (NCMAX * (TY / RY) + kNT - 1) / kNT;
+    constexpr int NCMAX = TX + (KXS > 0 ? KXS : KMAX) - 1;
+    constexpr int NRMAX = TY + (KYS > 0 ? KYS : KMAX) - 1;
+    constexpr int NLMAX = (NCMAX * NRMAX + NT - 1) / NT;
+    constexpr int NTASKMAX = (NCMAX * (TY / RY) + NT - 1) / NT;
 
-    const int NC = TX + a.kx - 1;
-    const int NR = TY + a.ky - 1;
+    const int kx = KXS > 0 ? KXS : a.kx;
+    const int ky = KYS > 0 ? KYS : a.ky;
+    const int NC = TX + kx - 1;
+    const int NR = TY + ky - 1;
     const int NCP = (NC + 3 + 3) & ~3;
     const int NIN = NR * NC;
     float* sIn = (float*)smem_raw;
@@ -164,7 +182,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     int32_t loff[NLMAX];
 #pragma unroll
     for (int k = 0; k < NLMAX; k++) {
-        const int i = tid + k * kNT;
+        const int i = tid + k * NT;
         loff[k] = 0;
         if (i < NIN) {
             const int r = i / NC;
@@ -178,7 +196,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     int tcol[NTASKMAX], trow[NTASKMAX];
 #pragma unroll
     for (int k = 0; k < NTASKMAX; k++) {
-        const int task = tid + k * kNT;
+        const int task = tid + k * NT;
         const int rg = task / NC;
         tcol[k] = task - rg * NC;
         trow[k] = rg * RY;
@@ -191,15 +209,17 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
 
     // slack reads must see finite values: zero the whole tile once
     const int lds_floats = (((NR + 3) * NC + 3) & ~3) + TY * NCP;
-    for (int i = tid; i < lds_floats; i += kNT) sIn[i] = 0.f;
+    for (int i = tid; i < lds_floats; i += NT) sIn[i] = 0.f;
 
-    float pre[NLMAX];
-    auto load_slice = [&](int s) {
+    // two register buffers: slice s+1 is in flight while slice s+2 is
+    // requested, so a load has two slice steps to arrive
+    float pre[NLMAX], pre2[NLMAX];
+    auto load_slice = [&](int s, float (&dstv)[NLMAX]) {
         const int zz = safe_map32(a.bz, a.Z, z0 - a.cz + s);
         const TS* base = src + (int64_t)zz * a.sz;
 #pragma unroll
         for (int k = 0; k < NLMAX; k++)
-            if (tid + k * kNT < NIN) pre[k] = load_s<TS>(base + loff[k]);
+            if (tid + k * NT < NIN) dstv[k] = load_s<TS>(base + loff[k]);
     };
 
     float ring[CPT][P];
@@ -208,24 +228,26 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
 #pragma unroll
         for (int p = 0; p < P; p++) ring[o][p] = 0.f;
 
-    load_slice(0);
-    __syncthreads(); // zero fill done before the first publish
-    for (int s = 0; s < nsl; s++) {
+    // one slice step with compile-time ring slot U: publish buffer `cur`
+    // (slice s), then refill it with slice s+2 while the other buffer's slice
+    // s+1 is still in flight
+    auto step = [&](auto slot, int s, float (&cur)[NLMAX]) {
+        constexpr int U = decltype(slot)::value;
 #pragma unroll
         for (int k = 0; k < NLMAX; k++)
-            if (tid + k * kNT < NIN) sIn[tid + k * kNT] = pre[k];
+            if (tid + k * NT < NIN) sIn[tid + k * NT] = cur[k];
         __syncthreads();
-        if (s + 1 < nsl) load_slice(s + 1);
+        if (s + 2 < nsl) load_slice(s + 2, cur);
 
         // y-pass: sIn -> sY
 #pragma unroll
         for (int k = 0; k < NTASKMAX; k++) {
-            if (tid + k * kNT < ntask) {
+            if (tid + k * NT < ntask) {
                 const float* col = sIn + trow[k] * NC + tcol[k];
                 float o[RY];
 #pragma unroll
                 for (int r = 0; r < RY; r++) o[r] = 0.f;
-                correlate<RY>(o, a.ky, a.wy, [&](int i) { return col[i * NC]; });
+                correlate<RY, KYS>(o, ky, a.wy, [&](int i) { return col[i * NC]; });
 #pragma unroll
                 for (int r = 0; r < RY; r++) sY[(trow[k] + r) * NCP + tcol[k]] = o[r];
             }
@@ -238,17 +260,33 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
         for (int o = 0; o < CPT; o++) xv[o] = 0.f;
         {
             const float* row = sY + ry * NCP + xs;
-            correlate<CPT>(xv, a.kx, a.wx, [&](int i) { return row[i]; });
+            correlate<CPT, KXS>(xv, kx, a.wx, [&](int i) { return row[i]; });
         }
 
         // z-ring
         const int zout = z0 + s - (P - 1);
         TS* out = dst + (int64_t)zout * a.dz + (int64_t)(y0 + ry) * a.dy + (x0 + xs);
-        ring_dispatch<0, P, CPT, TS>(s % P, ring, xv, s, a, out, row_ok, x0 + xs);
+        ring_step<U, P, CPT, TS>(ring, xv, s, a, out, row_ok, x0 + xs);
+    };
+
+    load_slice(0, pre);
+    if (nsl > 1) load_slice(1, pre2);
+    __syncthreads(); // zero fill done before the first publish
+    // 2P steps per iteration: slice s = s0 + u has ring slot u % P and
+    // prefetch buffer u % 2, both compile-time (s0 is a multiple of 2P)
+    for (int s0 = 0; s0 < nsl; s0 += 2 * P) {
+        static_for<2 * P>([&](auto ic) {
+            constexpr int u = decltype(ic)::value;
+            const int s = s0 + u;
+            if (s < nsl) {
+                if constexpr (u % 2 == 0) step(IC<u % P>{}, s, pre);
+                else step(IC<u % P>{}, s, pre2);
+            }
+        });
     }
 }
 
-template <class TS, int TX, int TY, int CPT, int P>
+template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P>
 __global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -259,7 +297,7 @@ __global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedAr
         const uint32_t per = a.ntiles >> 3;
         tile = (tile & 7u) * per + (tile >> 3);
     }
-    fused_tile<TS, TX, TY, CPT, P>(a, tile, smem_raw);
+    fused_tile<TS, TX, TY, CPT, KXS, KYS, P>(a, tile, smem_raw);
 }
 
 // Persistent work queue over many images with the same z tap count:
@@ -268,8 +306,8 @@ __global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedAr
 // tile_start[njobs] = total).
 template <class TS, int TX, int TY, int CPT, int P>
 __global__ __launch_bounds__(TX * TY / CPT) void fused_queue_kernel(const FusedArgs* __restrict__ jobs,
-                                                          const uint32_t* __restrict__ tile_start, int njobs,
-                                                          uint32_t total, uint32_t* counter)
+                                                                    const uint32_t* __restrict__ tile_start,
+                                                                    int njobs, uint32_t total, uint32_t* counter)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ uint32_t next;
@@ -286,7 +324,7 @@ __global__ __launch_bounds__(TX * TY / CPT) void fused_queue_kernel(const FusedA
             else hi = mid;
         }
         lo = __builtin_amdgcn_readfirstlane(lo);
-        fused_tile<TS, TX, TY, CPT, P>(jobs[lo], t - tile_start[lo], smem_raw);
+        fused_tile<TS, TX, TY, CPT, 0, 0, P>(jobs[lo], t - tile_start[lo], smem_raw);
         __syncthreads(); // LDS is reused by the next tile
     }
 }
@@ -296,43 +334,54 @@ __global__ void reset_counter_kernel(uint32_t* c) { *c = 0u; }
 // ------------------------------------------------------------- dispatch
 constexpr int kTX = 64, kTY = 16;
 
-template <class F> hipError_t with_ring(int ring, F&& f)
-{
-    switch (ring) {
-#define NDL_RING(R) case R: return f(std::integral_constant<int, R>{});
-        NDL_RING(1) NDL_RING(3) NDL_RING(5) NDL_RING(7) NDL_RING(9) NDL_RING(11) NDL_RING(13)
-        NDL_RING(15) NDL_RING(17) NDL_RING(19) NDL_RING(21) NDL_RING(23) NDL_RING(25)
-#undef NDL_RING
-    }
-    return hipErrorInvalidValue;
-}
-
-template <class F> hipError_t with_storage(int dt, F&& f)
-{
-    switch (dt) {
-    case NDL_F32: return f(float{});
-    case NDL_F16: return f(f16_t{});
-    case NDL_BF16: return f(bf16_t{});
-    case NDL_U8: return f(uint8_t{});
-    }
-    return hipErrorInvalidValue;
-}
-
-template <class F> hipError_t with_cpt(int cpt, F&& f)
-{
-    switch (cpt) {
-    case 1: return f(std::integral_constant<int, 1>{});
-    case 2: return f(std::integral_constant<int, 2>{});
-    }
-    return hipErrorInvalidValue;
-}
-
-template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, F&& f)
+// Instantiation table.  Specialised (compile-time taps) filters first, then
+// generic ring periods per storage type.  f(ts, cpt, kx, ky, ring) launches.
+template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int kx, int ky, F&& f)
 {
     if (cfg.tx != kTX || cfg.ty != kTY) return hipErrorInvalidValue;
-    return with_storage(dt, [&](auto ts) {
-        return with_cpt(cfg.cpt, [&](auto c) { return with_ring(cfg.ring, [&](auto r) { return f(ts, c, r); }); });
-    });
+    const int cpt = cfg.cpt, kz = cfg.ring;
+#define NDL_SPEC(DT, T, C, KX, KY, KZ)                                                        \
+    if (dt == DT && cpt == C && kx == KX && ky == KY && kz == KZ)                             \
+        return f(T{}, IC<C>{}, IC<KX>{}, IC<KY>{}, IC<KZ>{});
+    // 3-D Gaussian sigma (1.5, 1.5, 3.0): 11/11/19 taps
+    NDL_SPEC(NDL_F32, float, 2, 11, 11, 19) NDL_SPEC(NDL_F32, float, 4, 11, 11, 19)
+    // 2-D Gaussian sigma 2: 13 taps
+    NDL_SPEC(NDL_F32, float, 2, 13, 13, 1) NDL_SPEC(NDL_F32, float, 4, 13, 13, 1)
+    // video: sigma 1 (7 taps) on x, y and a 5-tap temporal binomial
+    NDL_SPEC(NDL_F32, float, 2, 7, 7, 5) NDL_SPEC(NDL_F32, float, 4, 7, 7, 5)
+    // 9-tap kernels on every axis, float and half-precision storage
+    NDL_SPEC(NDL_F32, float, 2, 9, 9, 9) NDL_SPEC(NDL_BF16, bf16_t, 2, 9, 9, 9) NDL_SPEC(NDL_F16, f16_t, 2, 9, 9, 9)
+    // uint8 7-tap box on x and y
+    NDL_SPEC(NDL_U8, uint8_t, 2, 7, 7, 1) NDL_SPEC(NDL_U8, uint8_t, 4, 7, 7, 1)
+#undef NDL_SPEC
+    if (cpt != 2) return hipErrorInvalidValue;
+#define NDL_GEN(DT, T, KZ) \
+    if (dt == DT && kz == KZ) return f(T{}, IC<2>{}, IC<0>{}, IC<0>{}, IC<KZ>{});
+    NDL_GEN(NDL_F32, float, 1) NDL_GEN(NDL_F32, float, 3) NDL_GEN(NDL_F32, float, 5) NDL_GEN(NDL_F32, float, 7)
+    NDL_GEN(NDL_F32, float, 9) NDL_GEN(NDL_F32, float, 11) NDL_GEN(NDL_F32, float, 13) NDL_GEN(NDL_F32, float, 15)
+    NDL_GEN(NDL_F32, float, 17) NDL_GEN(NDL_F32, float, 19) NDL_GEN(NDL_F32, float, 21) NDL_GEN(NDL_F32, float, 23)
+    NDL_GEN(NDL_F32, float, 25)
+    NDL_GEN(NDL_F16, f16_t, 1) NDL_GEN(NDL_F16, f16_t, 3) NDL_GEN(NDL_F16, f16_t, 5) NDL_GEN(NDL_F16, f16_t, 7)
+    NDL_GEN(NDL_F16, f16_t, 9) NDL_GEN(NDL_F16, f16_t, 11) NDL_GEN(NDL_F16, f16_t, 13)
+    NDL_GEN(NDL_BF16, bf16_t, 1) NDL_GEN(NDL_BF16, bf16_t, 3) NDL_GEN(NDL_BF16, bf16_t, 5)
+    NDL_GEN(NDL_BF16, bf16_t, 7) NDL_GEN(NDL_BF16, bf16_t, 9) NDL_GEN(NDL_BF16, bf16_t, 11)
+    NDL_GEN(NDL_BF16, bf16_t, 13)
+    NDL_GEN(NDL_U8, uint8_t, 1) NDL_GEN(NDL_U8, uint8_t, 3) NDL_GEN(NDL_U8, uint8_t, 5) NDL_GEN(NDL_U8, uint8_t, 7)
+#undef NDL_GEN
+    return hipErrorInvalidValue;
+}
+
+// queue kernels: float storage, generic taps
+template <class F> hipError_t with_queue_instance(const FusedConfig& cfg, int dt, F&& f)
+{
+    if (cfg.tx != kTX || cfg.ty != kTY || cfg.cpt != 2 || dt != NDL_F32) return hipErrorInvalidValue;
+    switch (cfg.ring) {
+#define NDL_Q(R) case R: return f(IC<R>{});
+        NDL_Q(1) NDL_Q(3) NDL_Q(5) NDL_Q(7) NDL_Q(9) NDL_Q(11) NDL_Q(13) NDL_Q(15) NDL_Q(17) NDL_Q(19) NDL_Q(21)
+        NDL_Q(23) NDL_Q(25)
+#undef NDL_Q
+    }
+    return hipErrorInvalidValue;
 }
 
 } // namespace
@@ -350,16 +399,23 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
     return (size_t)((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP) * 4;
 }
 
+bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue)
+{
+    if (queue) return with_queue_instance(cfg, dt, [](auto) { return hipSuccess; }) == hipSuccess;
+    return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
+}
+
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool, hipStream_t s)
 {
     if (a.ntiles == 0) return hipSuccess;
     if (cfg.ring != a.kz) return hipErrorInvalidValue;
     const size_t lds = fused_lds_bytes(a, cfg, false);
-    return with_instance(cfg, dt, [&](auto ts, auto c, auto r) {
+    return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r) {
         using TS = decltype(ts);
         constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
-        hipLaunchKernelGGL((fused_tile_kernel<TS, kTX, kTY, CPT, decltype(r)::value>), dim3(a.ntiles), dim3(NT), lds, s,
-                           a);
+        hipLaunchKernelGGL((fused_tile_kernel<TS, kTX, kTY, CPT, decltype(kxs)::value, decltype(kys)::value,
+                                              decltype(r)::value>),
+                           dim3(a.ntiles), dim3(NT), lds, s, a);
         return hipGetLastError();
     });
 }
@@ -367,24 +423,27 @@ hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool
 hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int dt, bool, size_t lds, hipStream_t s)
 {
     if (q.total == 0) return hipSuccess;
+    if (!fused_supported(cfg, dt, 0, 0, true)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, q.counter);
-    return with_instance(cfg, dt, [&](auto ts, auto c, auto r) {
-        using TS = decltype(ts);
-        constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
-        hipLaunchKernelGGL((fused_queue_kernel<TS, kTX, kTY, CPT, decltype(r)::value>), dim3(q.grid), dim3(NT), lds, s,
-                           q.jobs, q.tile_start, q.njobs, q.total, q.counter);
+    return with_queue_instance(cfg, dt, [&](auto r) {
+        constexpr int NT = kTX * kTY / 2;
+        hipLaunchKernelGGL((fused_queue_kernel<float, kTX, kTY, 2, decltype(r)::value>), dim3(q.grid), dim3(NT), lds,
+                           s, q.jobs, q.tile_start, q.njobs, q.total, q.counter);
         return hipGetLastError();
     });
 }
 
-int fused_blocks_per_cu(const FusedConfig& cfg, int dt, size_t lds)
+int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
     int n = 0;
-    hipError_t e = with_instance(cfg, dt, [&](auto ts, auto c, auto r) {
+    hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r) {
         using TS = decltype(ts);
         constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&fused_tile_kernel<TS, kTX, kTY, CPT, decltype(r)::value>), NT, lds);
+            &n,
+            reinterpret_cast<const void*>(&fused_tile_kernel<TS, kTX, kTY, CPT, decltype(kxs)::value,
+                                                             decltype(kys)::value, decltype(r)::value>),
+            NT, lds);
     });
     return e == hipSuccess ? n : 0;
 }

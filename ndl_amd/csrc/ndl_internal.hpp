@@ -114,7 +114,9 @@ hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int d
                               hipStream_t s);
 int fused_ring_for(int kz);       // ring period for kz, or -1 when unsupported
 // resident workgroups per CU for this instantiation (0 if unknown)
-int fused_blocks_per_cu(const FusedConfig& cfg, int dt, size_t lds);
+int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
+// whether an instantiation exists (queue: the persistent work-queue kernel)
+bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue);
 size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool acc8);
 
 } // namespace ndl

@@ -243,6 +243,22 @@ static void adjoints()
     }
 }
 
+// image_convolution_tests.cpp:117-160: ramp -> 1 in the interior; plane 2x+3y -> (2,3)
+static void gradients()
+{
+    OwnedImage<double, 2> plane({9, 7});
+    for (int y = 0; y < 7; y++)
+        for (int x = 0; x < 9; x++) plane(x, y) = 2.0 * x + 3.0 * y;
+    OwnedImage<double, 3> g({2, 9, 7});
+    gradient(plane, g);
+    bool ok = true;
+    for (int y = 1; y < 6; y++)
+        for (int x = 1; x < 8; x++) ok &= std::fabs(g(0, x, y) - 2.0) < 1e-9 && std::fabs(g(1, x, y) - 3.0) < 1e-9;
+    report("gradient() of the plane 2x+3y is (2,3) in the interior", ok);
+    // Reflect at the edge: x=0 sees x=-1 -> 0, so d/dx = 0.5*(x1 - x0)
+    report("gradient() Reflect border at x=0", std::fabs(g(0, 0, 3) - 0.5 * (plane(1, 3) - plane(0, 3))) < 1e-12);
+}
+
 int main()
 {
     int n = 0;
@@ -258,6 +274,7 @@ int main()
     views();
     per_axis_fp32();
     adjoints();
+    gradients();
     std::printf("%d failures\n", failures);
     return failures;
 }
