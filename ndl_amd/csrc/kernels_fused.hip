@@ -5,9 +5,11 @@
 // axis z, one input slice at a time:
 //   1. the slice's (TY+ky-1) x (TX+kx-1) input window, border indices
 //      resolved once per workgroup (detail.h:178-194 semantics), is requested
-//      two slices ahead into registers (two buffers) and published to LDS;
+//      two steps ahead into registers (two buffers) and published to LDS;
 //   2. y-pass: LDS -> LDS, 4 rows per task;
 //   3. x-pass: LDS -> registers, CPT consecutive x outputs per thread;
+//   steps are software-pipelined (y-pass of slice s beside the x/z-pass of
+//   slice s-1, double-buffered LDS) so each step needs one barrier;
 //   4. z-pass: each thread keeps the last P = kz x-filtered slices of its
 //      columns in a register ring; once kz slices are in, one output slice is
 //      written.  The slice loop is unrolled by 2P so every ring slot and
@@ -153,8 +155,13 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     const int NR = TY + ky - 1;
     const int NCP = (NC + 3 + 3) & ~3;
     const int NIN = NR * NC;
-    float* sIn = (float*)smem_raw;
-    float* sY = sIn + (((NR + 3) * NC + 3) & ~3);
+    // two input windows and two y-filtered planes (pipelined steps)
+    const int IN_FL = ((NR + 3) * NC + 3) & ~3;
+    const int Y_FL = TY * NCP;
+    float* sIn0 = (float*)smem_raw;
+    float* sIn1 = sIn0 + IN_FL;
+    float* sY0 = sIn1 + IN_FL;
+    float* sY1 = sY0 + Y_FL;
 
     uint32_t q = a.fdx.div(tile);
     const int tix = (int)(tile - q * (uint32_t)a.tiles_x);
@@ -208,14 +215,16 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     const bool row_ok = y0 + ry < a.Y;
 
     // slack reads must see finite values: zero the whole tile once
-    const int lds_floats = (((NR + 3) * NC + 3) & ~3) + TY * NCP;
-    for (int i = tid; i < lds_floats; i += NT) sIn[i] = 0.f;
+    const int lds_floats = 2 * (IN_FL + Y_FL);
+    for (int i = tid; i < lds_floats; i += NT) sIn0[i] = 0.f;
 
     // two register buffers: slice s+1 is in flight while slice s+2 is
     // requested, so a load has two slice steps to arrive
     float pre[NLMAX], pre2[NLMAX];
+    // the z border map only matters for chunks that reach past an end
+    const bool z_interior = z0 - a.cz >= 0 && z0 - a.cz + nsl <= a.Z;
     auto load_slice = [&](int s, float (&dstv)[NLMAX]) {
-        const int zz = safe_map32(a.bz, a.Z, z0 - a.cz + s);
+        const int zz = z_interior ? z0 - a.cz + s : safe_map32(a.bz, a.Z, z0 - a.cz + s);
         const TS* base = src + (int64_t)zz * a.sz;
 #pragma unroll
         for (int k = 0; k < NLMAX; k++)
@@ -228,60 +237,71 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
 #pragma unroll
         for (int p = 0; p < P; p++) ring[o][p] = 0.f;
 
-    // one slice step with compile-time ring slot U: publish buffer `cur`
-    // (slice s), then refill it with slice s+2 while the other buffer's slice
-    // s+1 is still in flight
-    auto step = [&](auto slot, int s, float (&cur)[NLMAX]) {
-        constexpr int U = decltype(slot)::value;
-#pragma unroll
-        for (int k = 0; k < NLMAX; k++)
-            if (tid + k * NT < NIN) sIn[tid + k * NT] = cur[k];
-        __syncthreads();
-        if (s + 2 < nsl) load_slice(s + 2, cur);
-
-        // y-pass: sIn -> sY
+    // y-pass of one slice: window `in` -> y-filtered plane `yp`
+    auto y_pass = [&](const float* in, float* yp) {
 #pragma unroll
         for (int k = 0; k < NTASKMAX; k++) {
             if (tid + k * NT < ntask) {
-                const float* col = sIn + trow[k] * NC + tcol[k];
+                const float* col = in + trow[k] * NC + tcol[k];
                 float o[RY];
 #pragma unroll
                 for (int r = 0; r < RY; r++) o[r] = 0.f;
                 correlate<RY, KYS>(o, ky, a.wy, [&](int i) { return col[i * NC]; });
 #pragma unroll
-                for (int r = 0; r < RY; r++) sY[(trow[k] + r) * NCP + tcol[k]] = o[r];
+                for (int r = 0; r < RY; r++) yp[(trow[k] + r) * NCP + tcol[k]] = o[r];
             }
         }
-        __syncthreads();
-
-        // x-pass: sY -> registers
+    };
+    // x-pass of one y-filtered plane, then ring slot U (slice s)
+    auto xz_pass = [&](auto slot, int s, const float* yp) {
+        constexpr int U = decltype(slot)::value;
         float xv[CPT];
 #pragma unroll
         for (int o = 0; o < CPT; o++) xv[o] = 0.f;
-        {
-            const float* row = sY + ry * NCP + xs;
-            correlate<CPT, KXS>(xv, kx, a.wx, [&](int i) { return row[i]; });
-        }
-
-        // z-ring
+        const float* row = yp + ry * NCP + xs;
+        correlate<CPT, KXS>(xv, kx, a.wx, [&](int i) { return row[i]; });
         const int zout = z0 + s - (P - 1);
         TS* out = dst + (int64_t)zout * a.dz + (int64_t)(y0 + ry) * a.dy + (x0 + xs);
         ring_step<U, P, CPT, TS>(ring, xv, s, a, out, row_ok, x0 + xs);
     };
+    auto publish = [&](const float (&v)[NLMAX], float* in) {
+#pragma unroll
+        for (int k = 0; k < NLMAX; k++)
+            if (tid + k * NT < NIN) in[tid + k * NT] = v[k];
+    };
 
+    // Pipelined schedule, one barrier per step.  Step s: publish slice s+1
+    // (its registers then fetch slice s+3), y-pass slice s, x/z-pass slice
+    // s-1.  Slice j lives in registers pre[j&1], LDS window sIn[j&1] and
+    // y-plane sY[j&1]; the barrier ends every read of the buffers the next
+    // step overwrites.
     load_slice(0, pre);
     if (nsl > 1) load_slice(1, pre2);
-    __syncthreads(); // zero fill done before the first publish
-    // 2P steps per iteration: slice s = s0 + u has ring slot u % P and
-    // prefetch buffer u % 2, both compile-time (s0 is a multiple of 2P)
-    for (int s0 = 0; s0 < nsl; s0 += 2 * P) {
-        static_for<2 * P>([&](auto ic) {
-            constexpr int u = decltype(ic)::value;
-            const int s = s0 + u;
-            if (s < nsl) {
-                if constexpr (u % 2 == 0) step(IC<u % P>{}, s, pre);
-                else step(IC<u % P>{}, s, pre2);
+    __syncthreads(); // zero fill done
+    publish(pre, sIn0);
+    if (nsl > 2) load_slice(2, pre);
+    __syncthreads();
+    auto step = [&](auto ic, int s) {
+        constexpr int u = decltype(ic)::value; // s = s0 + u, s0 a multiple of 2P
+        constexpr bool even = (u % 2) == 0;
+        if (s + 1 < nsl) {
+            if constexpr (even) {
+                publish(pre2, sIn1);
+                if (s + 3 < nsl) load_slice(s + 3, pre2);
+            } else {
+                publish(pre, sIn0);
+                if (s + 3 < nsl) load_slice(s + 3, pre);
             }
+        }
+        if (s < nsl) y_pass(even ? sIn0 : sIn1, even ? sY0 : sY1);
+        if (s >= 1) xz_pass(IC<(u + P - 1) % P>{}, s - 1, even ? sY1 : sY0);
+        __syncthreads();
+    };
+    // 2P steps per iteration keep ring slots and buffers compile-time
+    for (int s0 = 0; s0 <= nsl; s0 += 2 * P) {
+        static_for<2 * P>([&](auto ic) {
+            const int s = s0 + decltype(ic)::value;
+            if (s <= nsl) step(ic, s);
         });
     }
 }
@@ -396,7 +416,7 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
     const int NC = cfg.tx + a.kx - 1;
     const int NR = cfg.ty + a.ky - 1;
     const int NCP = (NC + 3 + 3) & ~3;
-    return (size_t)((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP) * 4;
+    return (size_t)2 * ((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP) * 4;
 }
 
 bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue)

@@ -307,3 +307,85 @@ def test_plan_matches_single_calls():
         single = gpu_sep(x, list(range(nd)), taps, [b] * nd, F32)
         np.testing.assert_array_equal(d.to_numpy(), single)
     ndl_amd.check(ndl_amd.lib().ndl_plan_destroy(plan))
+
+
+# ------------------------------------------------------- half-precision storage
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("border", ["clamp", "reflect"])
+def test_half_storage_forward_and_adjoint(storage, border):
+    # config-5 shape at 40^3: 9-tap random kernels, FP32 accumulation; the FP64
+    # oracle is fed the storage-rounded input, so the error is accumulation +
+    # one output rounding (2^-8 relative for bf16, 2^-11 for f16)
+    xs, xd, w, taps = workloads.config5(n=40, storage=storage)
+    bf = storage == "bf16"
+    d = ndl_amd.DeviceArray.from_numpy(xs, bf16=bf)
+    y = ndl_amd.DeviceArray(xs.shape, xs.dtype, bf16=bf)
+    ndl_amd.sepconv_dev(d, y, [0, 1, 2], taps, [border] * 3, F32)
+    got = y.to_numpy()
+    got = oracle.bf16_to_f64(got) if bf else got.astype(np.float64)
+    want = oracle.sepconv(xd, [0, 1, 2], taps, [border] * 3, dst=np.zeros(xd.shape))
+    ulp = 2.0 ** -8 if bf else 2.0 ** -11
+    scale = np.abs(xd).max() * np.prod([np.abs(t).sum() for t in taps])
+    assert np.all(np.abs(got - want) <= ulp * np.abs(want) + 1e-6 * scale)
+    # adjoint of the same operator on an f32 seed vs the FP64 transpose
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    gx = ndl_amd.DeviceArray(w.shape, np.float32)
+    ndl_amd.adjoint_input_dev(dw, gx, [0, 1, 2], taps, [border] * 3, F32)
+    want = oracle.adjoint_input(w.astype(np.float64), [0, 1, 2], taps, [border] * 3)
+    assert bound_err(gx.to_numpy(), want, w, taps) <= 2e-6
+
+
+# ---------------------------------------------------------- batched, uneven
+def test_plan_uneven_instances_vs_oracle():
+    # config-4 style: uneven 2-D/3-D instances, per-axis sigma, mixed borders
+    items = workloads.config4(count=24, n2d=16, max2d=300, max3d=48)
+    srcs, dsts, bitems, keep, wants = [], [], [], [], []
+    for i, (shape, taps, border) in enumerate(items):
+        x = workloads.config4_values(5, i, shape)
+        nd = len(shape)
+        wants.append(oracle.sepconv(x, list(range(nd)), taps, [border] * nd))
+        sd = ndl_amd.DeviceArray.from_numpy(x)
+        dd = ndl_amd.DeviceArray(shape, np.float32)
+        ks = abi.Kernels(taps, [border] * nd)
+        it = abi.BatchItem()
+        it.src, it.dst, it.naxes = sd.tensor(), dd.tensor(), nd
+        for a in range(nd):
+            it.axes[a] = a
+            it.k[a] = ks.arr[a]
+        srcs.append(sd)
+        dsts.append(dd)
+        bitems.append(it)
+        keep.append(ks)
+    import ctypes as C
+    arr = (abi.BatchItem * len(bitems))(*bitems)
+    plan = C.c_void_p()
+    ndl_amd.check(ndl_amd.lib().ndl_plan_create(arr, len(bitems), F32, C.byref(plan)))
+    for rep in range(3):  # graph replays, with host work and syncs between
+        for d in dsts:
+            ndl_amd.check(ndl_amd.lib().ndl_memset(d.ptr, 0, d.nbytes, None))
+        ndl_amd.check(ndl_amd.lib().ndl_plan_execute(plan, None))
+        ndl_amd.synchronize()
+        for d, want in zip(dsts, wants):
+            assert rel_err(d.to_numpy(), want) <= 1e-6, rep
+    ndl_amd.check(ndl_amd.lib().ndl_plan_destroy(plan))
+
+
+def test_u8_gaussian_rounding_bit_exact():
+    # gaussian_blur on uint8 with std::round (convolution.h:180-185), 3-D
+    rng = np.random.default_rng(21)
+    x = rng.integers(0, 256, size=(19, 23, 31), dtype=np.uint8)
+    for b in ["clamp", "wrap", "reflect"]:
+        want = oracle.gaussian_blur(x, 1.4, b)
+        np.testing.assert_array_equal(ndl_amd.gaussian_blur(x, 1.4, b), want)
+
+
+@pytest.mark.parametrize("taps7", [np.ones(7), np.array([1, 2, 3, 4, 3, 2, 1.0])])
+def test_u8_integer_taps_fused_bit_exact(taps7):
+    # integer taps on uint8 run the fused kernel (float accumulation, exact
+    # below 2^24) for planar data, the per-axis chain for interleaved data
+    rng = np.random.default_rng(22)
+    for shape, axes in [((3, 70, 133), [0, 1]), ((70, 133, 3), [1, 2]), ((9, 40, 50), [0, 1, 2])]:
+        x = rng.integers(0, 256, size=shape, dtype=np.uint8)
+        taps = [taps7] * len(axes)
+        want = oracle.sepconv(x, axes, taps, ["wrap"] * len(axes))
+        np.testing.assert_array_equal(gpu_sep(x, axes, taps, ["wrap"] * len(axes), REF), want)
