@@ -26,8 +26,19 @@ FLAGS = [
 ]
 
 
+def _stale(src, out):
+    """NDL_BUILD_INCREMENTAL=1 skips objects newer than every source and header."""
+    if os.environ.get("NDL_BUILD_INCREMENTAL") != "1" or not os.path.exists(out):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps.append(os.path.join(HERE, "..", "include", "ndl_gpu.h"))
+    return max(os.path.getmtime(d) for d in deps) > os.path.getmtime(out)
+
+
 def _compile(src):
     out = os.path.join(OBJ, src.replace(".hip", ".o"))
+    if not _stale(src, out):
+        return out
     cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

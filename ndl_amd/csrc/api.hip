@@ -329,6 +329,23 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
         if (src->stride[d] == 1 && dst->stride[d] == 1 && src->extent[d] > 1) { xd = d; break; }
     if (xd < 0) return false;
     role[xd] = 0;
+    // interleaved channels ({C, W, H} colour, per_channel() slices with
+    // x-stride C, image/kernels.h:78-87): a short untouched unit-stride
+    // dimension followed by a convolved dimension of stride C merge into one
+    // tile row of C*W elements whose x taps are C elements apart
+    int il = 1, chd = -1;
+    if (conv_of_dim[xd] < 0 && src->extent[xd] <= 4 && dt == NDL_U8) {
+        for (int d = 0; d < nd; d++)
+            if (d != xd && conv_of_dim[d] >= 0 && src->stride[d] == src->extent[xd] &&
+                dst->stride[d] == dst->extent[xd]) {
+                il = (int)src->extent[xd];
+                chd = xd;
+                xd = d;
+                role[chd] = 3;
+                role[xd] = 0;
+                break;
+            }
+    }
     // y = the convolved non-x dimension with the smaller stride, z = the
     // other: every pass order gives the same tile (FP32 accumulation and the
     // exact integer path do not depend on pass order)
@@ -346,7 +363,8 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     memset(&fa, 0, sizeof fa);
     fa.src = src->data;
     fa.dst = dst->data;
-    fa.X = (int32_t)src->extent[xd];
+    if (src->extent[xd] * il > 0x7fffffff) return false;
+    fa.X = (int32_t)(src->extent[xd] * il);
     fa.Y = yd >= 0 ? (int32_t)src->extent[yd] : 1;
     fa.Z = zd >= 0 ? (int32_t)src->extent[zd] : 1;
     fa.sy = yd >= 0 ? src->stride[yd] : 0;
@@ -390,6 +408,7 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     cfg.ty = 16;
     cfg.cpt = env_int("NDL_FUSED_CPT", 2);
     if (cfg.cpt != 2 && cfg.cpt != 4) cfg.cpt = 2;
+    cfg.il = il;
     cfg.ring = fused_ring_for(fa.kz);
     if (cfg.ring < 0) return false;
     if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) {
@@ -826,9 +845,12 @@ NDL_API int ndl_version(void) { return 100; }
 // ---------------------------------------------------------------- batched
 // A plan groups its fused-eligible items by storage type and z tap count
 // (the register ring's compile-time period) into device job tables;
-// execute() runs one persistent work-queue launch per group
-// (captured once into a hipGraph and replayed), then any remaining items
-// through the per-axis chain.
+// execute() runs one persistent work-queue launch per group, then any
+// remaining items through the per-axis chain.  The z-chunk length of the
+// jobs is autotuned: every candidate is built at creation, the first
+// execute times each with HIP events on the caller's stream and keeps the
+// fastest, whose launches are then captured once into a hipGraph and
+// replayed.
 namespace {
 struct PlanGroup {
     int dt = 0;
@@ -840,13 +862,27 @@ struct PlanGroup {
     void* d_jobs = nullptr;
     void* d_ts = nullptr;
 };
+struct PlanVariant {
+    int tz = 0;                 // z-chunk candidate
+    std::vector<PlanGroup> groups;
+    float ms = -1.f;            // measured time of one execute
+    void release()
+    {
+        for (auto& g : groups) {
+            if (g.d_jobs) (void)hipFree(g.d_jobs);
+            if (g.d_ts) (void)hipFree(g.d_ts);
+            g.d_jobs = g.d_ts = nullptr;
+        }
+    }
+};
 } // namespace
 
 struct ndl_plan {
     std::vector<ndl_batch_item> items;
     std::vector<std::vector<double>> taps; // owned copies, NDL_MAX_DIM per item
     int32_t accum = NDL_ACCUM_F32;
-    std::vector<PlanGroup> groups;
+    std::vector<PlanVariant> variants;     // autotuning candidates
+    int chosen = -1;
     std::vector<int> fallback;
     uint32_t* d_counter = nullptr;
     uint32_t grid = 1024;
@@ -859,19 +895,16 @@ struct ndl_plan {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
-        for (auto& g : groups) {
-            if (g.d_jobs) (void)hipFree(g.d_jobs);
-            if (g.d_ts) (void)hipFree(g.d_ts);
-        }
+        for (auto& v : variants) v.release();
         if (d_counter) (void)hipFree(d_counter);
     }
 };
 
 namespace {
-int launch_groups(ndl_plan* p, hipStream_t s)
+int launch_groups(ndl_plan* p, const PlanVariant& v, hipStream_t s)
 {
-    for (size_t i = 0; i < p->groups.size(); i++) {
-        PlanGroup& g = p->groups[i];
+    for (size_t i = 0; i < v.groups.size(); i++) {
+        const PlanGroup& g = v.groups[i];
         FusedQueue q;
         q.jobs = (const FusedArgs*)g.d_jobs;
         q.tile_start = (const uint32_t*)g.d_ts;
@@ -882,6 +915,89 @@ int launch_groups(ndl_plan* p, hipStream_t s)
         hipError_t e = launch_fused_queue(q, g.cfg, g.dt, g.int_acc, g.lds, s);
         if (e != hipSuccess) return hip_fail(e, "work-queue launch");
     }
+    return NDL_OK;
+}
+
+// one candidate: jobs with z chunks of at most tz, most expensive tiles first
+int build_variant(ndl_plan* p, int tz, PlanVariant& v, bool collect_fallback)
+{
+    struct Cand { int item; FusedArgs fa; FusedConfig cfg; bool int_acc; double cost; };
+    std::vector<Cand> cands;
+    v.tz = tz;
+    for (int i = 0; i < (int)p->items.size(); i++) {
+        ndl_batch_item& it = p->items[i];
+        if (numel(&it.src) == 0) continue;
+        Cand c;
+        c.item = i;
+        if (plan_fused(&it.src, &it.dst, it.naxes, it.axes, it.k, p->accum, c.fa, c.cfg, c.int_acc, tz)) {
+            c.fa.xcd_remap = 0;
+            c.cost = (double)(std::min(c.fa.tz, c.fa.Z) + c.fa.kz - 1);
+            cands.push_back(c);
+        } else if (collect_fallback) {
+            p->fallback.push_back(i);
+        }
+    }
+    std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.cost > b.cost; });
+    for (auto& c : cands) {
+        int dt = p->items[c.item].src.dtype;
+        PlanGroup* g = nullptr;
+        for (auto& gg : v.groups)
+            if (gg.dt == dt && gg.int_acc == c.int_acc && gg.cfg.ring == c.cfg.ring) g = &gg;
+        if (!g) {
+            v.groups.emplace_back();
+            g = &v.groups.back();
+            g->dt = dt;
+            g->int_acc = c.int_acc;
+            g->cfg = c.cfg;
+            g->tile_start.push_back(0);
+        }
+        g->lds = std::max(g->lds, fused_lds_bytes(c.fa, c.cfg, false));
+        g->jobs.push_back(c.fa);
+        g->tile_start.push_back(g->tile_start.back() + c.fa.ntiles);
+    }
+    hipError_t e;
+    for (auto& g : v.groups) {
+        size_t jb = g.jobs.size() * sizeof(FusedArgs), tb = g.tile_start.size() * sizeof(uint32_t);
+        if ((e = hipMalloc(&g.d_jobs, jb)) != hipSuccess || (e = hipMalloc(&g.d_ts, tb)) != hipSuccess)
+            return fail(NDL_ERR_NOMEM, "plan tables: %s", hipGetErrorString(e));
+        if ((e = hipMemcpy(g.d_jobs, g.jobs.data(), jb, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(g.d_ts, g.tile_start.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, "plan table upload");
+    }
+    return NDL_OK;
+}
+
+// first execute: time every candidate (one warm-up, three timed runs) and
+// keep the fastest
+int autotune(ndl_plan* p, hipStream_t s)
+{
+    if (p->variants.size() == 1) {
+        p->chosen = 0;
+        return NDL_OK;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        p->chosen = 0;
+        return NDL_OK;
+    }
+    int best = 0;
+    for (int v = 0; v < (int)p->variants.size(); v++) {
+        int rc = launch_groups(p, p->variants[v], s);
+        if (rc) return rc;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < 3; r++)
+            if ((rc = launch_groups(p, p->variants[v], s))) return rc;
+        (void)hipEventRecord(e1, s);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
+        p->variants[v].ms = ms / 3;
+        if (p->variants[v].ms < p->variants[best].ms) best = v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->chosen = best;
+    for (int v = 0; v < (int)p->variants.size(); v++)
+        if (v != best) p->variants[v].release();
     return NDL_OK;
 }
 } // namespace
@@ -906,54 +1022,22 @@ NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accu
             it.k[a].taps = v.data();
         }
     }
-    // fused-eligible items, grouped by storage type; most expensive tiles first
-    struct Cand { int item; FusedArgs fa; FusedConfig cfg; bool int_acc; double cost; };
-    std::vector<Cand> cands;
-    const int tz_plan = env_int("NDL_PLAN_TZ", 64);
-    for (int i = 0; i < n; i++) {
-        ndl_batch_item& it = p->items[i];
-        if (numel(&it.src) == 0) continue;
-        Cand c;
-        c.item = i;
-        if (plan_fused(&it.src, &it.dst, it.naxes, it.axes, it.k, accum, c.fa, c.cfg, c.int_acc, tz_plan)) {
-            c.fa.xcd_remap = 0;
-            c.cost = (double)(std::min(c.fa.tz, c.fa.Z) + c.fa.kz - 1);
-            cands.push_back(c);
-        } else {
-            p->fallback.push_back(i);
-        }
-    }
-    std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.cost > b.cost; });
-    for (auto& c : cands) {
-        int dt = p->items[c.item].src.dtype;
-        PlanGroup* g = nullptr;
-        for (auto& gg : p->groups)
-            if (gg.dt == dt && gg.int_acc == c.int_acc && gg.cfg.ring == c.cfg.ring) g = &gg;
-        if (!g) {
-            p->groups.emplace_back();
-            g = &p->groups.back();
-            g->dt = dt;
-            g->int_acc = c.int_acc;
-            g->cfg = c.cfg;
-            g->tile_start.push_back(0);
-        }
-        g->lds = std::max(g->lds, fused_lds_bytes(c.fa, c.cfg, false));
-        g->jobs.push_back(c.fa);
-        g->tile_start.push_back(g->tile_start.back() + c.fa.ntiles);
+    std::vector<int> tzs;
+    int forced = env_int("NDL_PLAN_TZ", 0);
+    if (forced > 0) tzs.push_back(forced);
+    else tzs = {32, 64, 128, 1 << 30};
+    size_t max_groups = 1;
+    for (size_t i = 0; i < tzs.size(); i++) {
+        p->variants.emplace_back();
+        int rc = build_variant(p.get(), tzs[i], p->variants.back(), i == 0);
+        if (rc) return rc;
+        max_groups = std::max(max_groups, p->variants.back().groups.size());
     }
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     p->grid = (uint32_t)std::max(1, cus) * (uint32_t)env_int("NDL_PLAN_WG_PER_CU", 4);
-    hipError_t e = hipMalloc((void**)&p->d_counter, sizeof(uint32_t) * std::max<size_t>(1, p->groups.size()));
+    hipError_t e = hipMalloc((void**)&p->d_counter, sizeof(uint32_t) * max_groups);
     if (e != hipSuccess) return fail(NDL_ERR_NOMEM, "plan counter: %s", hipGetErrorString(e));
-    for (auto& g : p->groups) {
-        size_t jb = g.jobs.size() * sizeof(FusedArgs), tb = g.tile_start.size() * sizeof(uint32_t);
-        if ((e = hipMalloc(&g.d_jobs, jb)) != hipSuccess || (e = hipMalloc(&g.d_ts, tb)) != hipSuccess)
-            return fail(NDL_ERR_NOMEM, "plan tables: %s", hipGetErrorString(e));
-        if ((e = hipMemcpy(g.d_jobs, g.jobs.data(), jb, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(g.d_ts, g.tile_start.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
-            return hip_fail(e, "plan table upload");
-    }
     *plan = p.release();
     return NDL_OK;
 }
@@ -962,13 +1046,18 @@ NDL_API int ndl_plan_execute(ndl_plan* p, void* stream)
 {
     if (!p) return fail(NDL_ERR_ARG, "null plan");
     hipStream_t s = (hipStream_t)stream;
-    if (!p->groups.empty()) {
+    if (p->chosen < 0) {
+        int rc = autotune(p, s);
+        if (rc) return rc;
+    }
+    const PlanVariant& v = p->variants[p->chosen];
+    if (!v.groups.empty()) {
         if (!p->tried_graph && env_int("NDL_PLAN_GRAPH", 1)) {
             // capture the work-queue launches once on a private stream
             p->tried_graph = true;
             if (hipStreamCreate(&p->cap) == hipSuccess &&
                 hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                int rc = launch_groups(p, p->cap);
+                int rc = launch_groups(p, v, p->cap);
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamEndCapture(p->cap, &g);
                 if (rc == NDL_OK && e == hipSuccess && g &&
@@ -985,7 +1074,7 @@ NDL_API int ndl_plan_execute(ndl_plan* p, void* stream)
             hipError_t e = hipGraphLaunch(p->exec, s);
             if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
         } else {
-            int rc = launch_groups(p, s);
+            int rc = launch_groups(p, v, s);
             if (rc) return rc;
         }
     }

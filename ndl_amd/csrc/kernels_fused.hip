@@ -59,30 +59,32 @@ template <int N, class F> __device__ __forceinline__ void static_for(F&& f)
 }
 
 // acc[o] += sum_{j<NTAP} w[j] * win[o + j], o < NO, taps ascending
-template <int NO, int NTAP, int NW>
+// (taps IL window elements apart: interleaved channels, IL = channel count)
+template <int NO, int NTAP, int NW, int IL = 1>
 __device__ __forceinline__ void fma_block(float (&acc)[NO], const float (&win)[NW], const float* w)
 {
-    static_assert(NO + NTAP - 1 <= NW, "window too short");
+    static_assert(NO + (NTAP - 1) * IL <= NW, "window too short");
 #pragma unroll
     for (int j = 0; j < NTAP; j++) {
         const float wj = w[j];
 #pragma unroll
-        for (int o = 0; o < NO; o++) acc[o] = __builtin_fmaf(win[o + j], wj, acc[o]);
+        for (int o = 0; o < NO; o++) acc[o] = __builtin_fmaf(win[o + j * IL], wj, acc[o]);
     }
 }
 
 // 1-D correlation of NO consecutive outputs read through get(i) (window
 // element i).  KS > 0: KS taps, fully unrolled.  KS == 0: K taps at run time,
 // 4-tap chunks then a 1..3-tap tail.
-template <int NO, int KS, class Get>
+template <int NO, int KS, int IL = 1, class Get>
 __device__ __forceinline__ void correlate(float (&acc)[NO], int K, const float* w, Get get)
 {
+    static_assert(KS > 0 || IL == 1, "interleaved taps need a compile-time tap count");
     if constexpr (KS > 0) {
-        constexpr int NW = NO + KS - 1;
+        constexpr int NW = NO + (KS - 1) * IL;
         float win[NW];
 #pragma unroll
         for (int i = 0; i < NW; i++) win[i] = get(i);
-        fma_block<NO, KS, NW>(acc, win, w);
+        fma_block<NO, KS, NW, IL>(acc, win, w);
     } else {
         constexpr int NW = NO + 3;
         int t0 = 0;
@@ -135,7 +137,7 @@ __device__ __forceinline__ void ring_step(float (&ring)[CPT][P], const float (&x
 // LDS (floats): sIn = (NR+3) x NC input window (+3 rows of slack for the
 // chunked y reads), sY = TY x NCP y-filtered rows (NCP >= NC+3, a multiple
 // of 4, slack for the chunked x reads).
-template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P>
+template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL = 1>
 __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, unsigned char* smem_raw)
 {
     constexpr int NT = TX * TY / CPT;
@@ -144,14 +146,14 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     constexpr int RY = 4;
     static_assert(TY % RY == 0, "TY must be a multiple of 4");
     constexpr int KMAX = 25;
-    constexpr int NCMAX = TX + (KXS > 0 ? KXS : KMAX) - 1;
+    constexpr int NCMAX = TX + ((KXS > 0 ? KXS : KMAX) - 1) * IL;
     constexpr int NRMAX = TY + (KYS > 0 ? KYS : KMAX) - 1;
     constexpr int NLMAX = (NCMAX * NRMAX + NT - 1) / NT;
     constexpr int NTASKMAX = (NCMAX * (TY / RY) + NT - 1) / NT;
 
     const int kx = KXS > 0 ? KXS : a.kx;
     const int ky = KYS > 0 ? KYS : a.ky;
-    const int NC = TX + kx - 1;
+    const int NC = TX + (kx - 1) * IL; // window columns (x taps IL apart)
     const int NR = TY + ky - 1;
     const int NCP = (NC + 3 + 3) & ~3;
     const int NIN = NR * NC;
@@ -195,7 +197,15 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
             const int r = i / NC;
             const int c = i - r * NC;
             const int yy = safe_map32(a.by, a.Y, y0 - a.cy + r);
-            const int xx = safe_map32(a.bx, a.X, x0 - a.cx + c);
+            int xx;
+            if constexpr (IL == 1) {
+                xx = safe_map32(a.bx, a.X, x0 - a.cx + c);
+            } else {
+                // merged (channel, x) index: map x, keep the channel
+                const int e = x0 - a.cx * IL + c;
+                const int w = e >= 0 ? e / IL : -((-e + IL - 1) / IL);
+                xx = safe_map32(a.bx, a.X / IL, w) * IL + (e - w * IL);
+            }
             loff[k] = (int32_t)((int64_t)yy * a.sy + xx);
         }
     }
@@ -259,7 +269,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
 #pragma unroll
         for (int o = 0; o < CPT; o++) xv[o] = 0.f;
         const float* row = yp + ry * NCP + xs;
-        correlate<CPT, KXS>(xv, kx, a.wx, [&](int i) { return row[i]; });
+        correlate<CPT, KXS, IL>(xv, kx, a.wx, [&](int i) { return row[i]; });
         const int zout = z0 + s - (P - 1);
         TS* out = dst + (int64_t)zout * a.dz + (int64_t)(y0 + ry) * a.dy + (x0 + xs);
         ring_step<U, P, CPT, TS>(ring, xv, s, a, out, row_ok, x0 + xs);
@@ -306,7 +316,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     }
 }
 
-template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P>
+template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL>
 __global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -317,7 +327,7 @@ __global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedAr
         const uint32_t per = a.ntiles >> 3;
         tile = (tile & 7u) * per + (tile >> 3);
     }
-    fused_tile<TS, TX, TY, CPT, KXS, KYS, P>(a, tile, smem_raw);
+    fused_tile<TS, TX, TY, CPT, KXS, KYS, P, IL>(a, tile, smem_raw);
 }
 
 // Persistent work queue over many images with the same z tap count:
@@ -359,10 +369,11 @@ constexpr int kTX = 64, kTY = 16;
 template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int kx, int ky, F&& f)
 {
     if (cfg.tx != kTX || cfg.ty != kTY) return hipErrorInvalidValue;
-    const int cpt = cfg.cpt, kz = cfg.ring;
-#define NDL_SPEC(DT, T, C, KX, KY, KZ)                                                        \
-    if (dt == DT && cpt == C && kx == KX && ky == KY && kz == KZ)                             \
-        return f(T{}, IC<C>{}, IC<KX>{}, IC<KY>{}, IC<KZ>{});
+    const int cpt = cfg.cpt, kz = cfg.ring, il = cfg.il;
+#define NDL_SPEC_IL(DT, T, C, KX, KY, KZ, IL)                                                 \
+    if (dt == DT && cpt == C && kx == KX && ky == KY && kz == KZ && il == IL)                 \
+        return f(T{}, IC<C>{}, IC<KX>{}, IC<KY>{}, IC<KZ>{}, IC<IL>{});
+#define NDL_SPEC(DT, T, C, KX, KY, KZ) NDL_SPEC_IL(DT, T, C, KX, KY, KZ, 1)
     // 3-D Gaussian sigma (1.5, 1.5, 3.0): 11/11/19 taps
     NDL_SPEC(NDL_F32, float, 2, 11, 11, 19) NDL_SPEC(NDL_F32, float, 4, 11, 11, 19)
     // 2-D Gaussian sigma 2: 13 taps
@@ -371,12 +382,14 @@ template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int 
     NDL_SPEC(NDL_F32, float, 2, 7, 7, 5) NDL_SPEC(NDL_F32, float, 4, 7, 7, 5)
     // 9-tap kernels on every axis, float and half-precision storage
     NDL_SPEC(NDL_F32, float, 2, 9, 9, 9) NDL_SPEC(NDL_BF16, bf16_t, 2, 9, 9, 9) NDL_SPEC(NDL_F16, f16_t, 2, 9, 9, 9)
-    // uint8 7-tap box on x and y
+    // uint8 7-tap box on x and y, planar and 3-channel interleaved
     NDL_SPEC(NDL_U8, uint8_t, 2, 7, 7, 1) NDL_SPEC(NDL_U8, uint8_t, 4, 7, 7, 1)
+    NDL_SPEC_IL(NDL_U8, uint8_t, 2, 7, 7, 1, 3)
 #undef NDL_SPEC
-    if (cpt != 2) return hipErrorInvalidValue;
+#undef NDL_SPEC_IL
+    if (cpt != 2 || il != 1) return hipErrorInvalidValue;
 #define NDL_GEN(DT, T, KZ) \
-    if (dt == DT && kz == KZ) return f(T{}, IC<2>{}, IC<0>{}, IC<0>{}, IC<KZ>{});
+    if (dt == DT && kz == KZ) return f(T{}, IC<2>{}, IC<0>{}, IC<0>{}, IC<KZ>{}, IC<1>{});
     NDL_GEN(NDL_F32, float, 1) NDL_GEN(NDL_F32, float, 3) NDL_GEN(NDL_F32, float, 5) NDL_GEN(NDL_F32, float, 7)
     NDL_GEN(NDL_F32, float, 9) NDL_GEN(NDL_F32, float, 11) NDL_GEN(NDL_F32, float, 13) NDL_GEN(NDL_F32, float, 15)
     NDL_GEN(NDL_F32, float, 17) NDL_GEN(NDL_F32, float, 19) NDL_GEN(NDL_F32, float, 21) NDL_GEN(NDL_F32, float, 23)
@@ -394,7 +407,7 @@ template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int 
 // queue kernels: float storage, generic taps
 template <class F> hipError_t with_queue_instance(const FusedConfig& cfg, int dt, F&& f)
 {
-    if (cfg.tx != kTX || cfg.ty != kTY || cfg.cpt != 2 || dt != NDL_F32) return hipErrorInvalidValue;
+    if (cfg.tx != kTX || cfg.ty != kTY || cfg.cpt != 2 || cfg.il != 1 || dt != NDL_F32) return hipErrorInvalidValue;
     switch (cfg.ring) {
 #define NDL_Q(R) case R: return f(IC<R>{});
         NDL_Q(1) NDL_Q(3) NDL_Q(5) NDL_Q(7) NDL_Q(9) NDL_Q(11) NDL_Q(13) NDL_Q(15) NDL_Q(17) NDL_Q(19) NDL_Q(21)
@@ -413,7 +426,7 @@ int fused_ring_for(int kz)
 
 size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
 {
-    const int NC = cfg.tx + a.kx - 1;
+    const int NC = cfg.tx + (a.kx - 1) * cfg.il;
     const int NR = cfg.ty + a.ky - 1;
     const int NCP = (NC + 3 + 3) & ~3;
     return (size_t)2 * ((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP) * 4;
@@ -422,7 +435,7 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
 bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue)
 {
     if (queue) return with_queue_instance(cfg, dt, [](auto) { return hipSuccess; }) == hipSuccess;
-    return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
+    return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
 }
 
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool, hipStream_t s)
@@ -430,11 +443,11 @@ hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool
     if (a.ntiles == 0) return hipSuccess;
     if (cfg.ring != a.kz) return hipErrorInvalidValue;
     const size_t lds = fused_lds_bytes(a, cfg, false);
-    return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r) {
+    return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il) {
         using TS = decltype(ts);
         constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
         hipLaunchKernelGGL((fused_tile_kernel<TS, kTX, kTY, CPT, decltype(kxs)::value, decltype(kys)::value,
-                                              decltype(r)::value>),
+                                              decltype(r)::value, decltype(il)::value>),
                            dim3(a.ntiles), dim3(NT), lds, s, a);
         return hipGetLastError();
     });
@@ -456,13 +469,14 @@ hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int d
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
     int n = 0;
-    hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r) {
+    hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il) {
         using TS = decltype(ts);
         constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &n,
             reinterpret_cast<const void*>(&fused_tile_kernel<TS, kTX, kTY, CPT, decltype(kxs)::value,
-                                                             decltype(kys)::value, decltype(r)::value>),
+                                                             decltype(kys)::value, decltype(r)::value,
+                                                             decltype(il)::value>),
             NT, lds);
     });
     return e == hipSuccess ? n : 0;

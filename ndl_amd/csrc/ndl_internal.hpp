@@ -97,6 +97,7 @@ struct FusedArgs {
 struct FusedConfig {
     int tx, ty, ring, kmax;       // template selection
     int cpt;                      // x outputs per thread (threads = tx*ty/cpt)
+    int il;                       // interleave: x taps il elements apart (channels)
 };
 // Returns hipErrorInvalidValue when no instantiation matches cfg/dtype.
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool int_acc,
