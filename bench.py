@@ -6,7 +6,9 @@ volume, U[0,1) from a counter-based seed, separable Gaussian
 sigma = (1.5, 1.5, 3.0) -> 11/11/19 taps, reflect border, FP32 accumulation.
 One step = forward (ndl_sepconv_forward) + input adjoint
 (ndl_sepconv_adjoint_input) of a seeded N(0,1) float32 field, inputs resident
-in HBM.  Each launch is timed with HIP events on the stream it runs on; by
+in HBM.  Each launch is timed with HIP events (created with
+hipEventDisableSystemFence, so the interval is the kernel plus its dispatch,
+without a system-scope release) on the stream it runs on; by
 default the Infinity Cache is flushed with a 1 GiB read before every step
 (cold protocol; --warm skips the flush and is reported as warm_value).
 
@@ -105,7 +107,7 @@ def main():
             raise RuntimeError("flush launch failed")
 
     def run(steps, cold):
-        ev = [[hip.event() for _ in range(3)] for _ in range(steps)]
+        ev = [[hip.event(hip.EVENT_NO_SYSTEM_FENCE) for _ in range(3)] for _ in range(steps)]
         ndl_amd.synchronize(stream)
         t0 = time.perf_counter()
         for i in range(steps):
@@ -137,7 +139,7 @@ def main():
 
     # measured streaming-copy ceiling (1 GiB -> 1 GiB... half each way)
     half = flush_bytes // 2
-    e0, e1 = hip.event(), hip.event()
+    e0, e1 = hip.event(hip.EVENT_NO_SYSTEM_FENCE), hip.event(hip.EVENT_NO_SYSTEM_FENCE)
     bu.ndl_bench_copy(dflush.ptr, dflush.ptr + half, half, stream)
     hip.record(e0, stream)
     for _ in range(5):
@@ -235,7 +237,7 @@ def _time_launches(hip, stream, fn, reps, flush=None):
     for _ in range(reps):
         if flush:
             flush()
-        e0, e1 = hip.event(), hip.event()
+        e0, e1 = hip.event(hip.EVENT_NO_SYSTEM_FENCE), hip.event(hip.EVENT_NO_SYSTEM_FENCE)
         hip.record(e0, stream)
         fn()
         hip.record(e1, stream)

@@ -257,7 +257,8 @@ class _Hip:
     def __init__(self):
         lib()
         self.h = C.CDLL("libamdhip64.so.7")
-        for n, a in [("hipEventCreate", [_P(_VP)]), ("hipEventRecord", [_VP, _VP]),
+        for n, a in [("hipEventCreate", [_P(_VP)]), ("hipEventCreateWithFlags", [_P(_VP), C.c_uint]),
+                     ("hipEventRecord", [_VP, _VP]),
                      ("hipEventSynchronize", [_VP]), ("hipEventDestroy", [_VP]),
                      ("hipEventElapsedTime", [_P(C.c_float), _VP, _VP]),
                      ("hipStreamCreate", [_P(_VP)]), ("hipStreamDestroy", [_VP]),
@@ -270,9 +271,13 @@ class _Hip:
         if rc != 0:
             raise RuntimeError(f"{what} failed with hipError {rc}")
 
-    def event(self):
+    # hipEventDisableSystemFence: the event does not wait for a system-scope
+    # release of the preceding work, only for its completion on the device
+    EVENT_NO_SYSTEM_FENCE = 0x20000000
+
+    def event(self, flags=0):
         e = _VP()
-        self.ok(self.h.hipEventCreate(C.byref(e)), "hipEventCreate")
+        self.ok(self.h.hipEventCreateWithFlags(C.byref(e), flags), "hipEventCreateWithFlags")
         return e.value
 
     def record(self, ev, stream=None):

@@ -224,9 +224,20 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     const int xs = (tid - ry * XG) * CPT;
     const bool row_ok = y0 + ry < a.Y;
 
-    // slack reads must see finite values: zero the whole tile once
-    const int lds_floats = 2 * (IN_FL + Y_FL);
-    for (int i = tid; i < lds_floats; i += NT) sIn0[i] = 0.f;
+    // the run-time-tap path reads up to 3 rows / columns past a window in
+    // 4-tap chunks: that slack must hold finite values.  Compile-time taps
+    // read exactly the window.
+    if constexpr (KXS == 0 || KYS == 0) {
+        for (int i = tid; i < 3 * NC; i += NT) {
+            sIn0[NIN + i] = 0.f;
+            sIn1[NIN + i] = 0.f;
+        }
+        for (int i = tid; i < TY * (NCP - NC); i += NT) {
+            const int r = i / (NCP - NC), c = NC + i - r * (NCP - NC);
+            sY0[r * NCP + c] = 0.f;
+            sY1[r * NCP + c] = 0.f;
+        }
+    }
 
     // two register buffers: slice s+1 is in flight while slice s+2 is
     // requested, so a load has two slice steps to arrive
@@ -287,7 +298,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     // step overwrites.
     load_slice(0, pre);
     if (nsl > 1) load_slice(1, pre2);
-    __syncthreads(); // zero fill done
+    __syncthreads(); // slack fill done
     publish(pre, sIn0);
     if (nsl > 2) load_slice(2, pre);
     __syncthreads();
