@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libndl_gpu.so")
 BENCH_LIB = os.path.join(HERE, "libndl_benchutil.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["kernels_generic.hip", "kernels_fused.hip", "api.hip"]
+SOURCES = ["kernels_fused.hip", "kernels_fused_edge_a.hip", "kernels_fused_edge_b.hip", "kernels_fused_edge_c.hip",
+           "kernels_fused_edge.hip", "kernels_generic.hip", "api.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -48,7 +49,7 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

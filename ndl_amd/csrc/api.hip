@@ -272,9 +272,9 @@ int device_cus()
 int occupancy(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int, int, size_t>, int> cache;
+    static std::map<std::tuple<int, int, int, int, int, int, int, size_t>, int> cache;
     std::lock_guard<std::mutex> lock(mu);
-    auto key = std::make_tuple(dt, cfg.cpt, cfg.ring, kx, ky, lds);
+    auto key = std::make_tuple(dt, cfg.cpt, cfg.ring, cfg.il, cfg.edge, kx, ky, lds);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int b = fused_blocks_per_cu(cfg, dt, kx, ky, lds);
@@ -284,10 +284,13 @@ int occupancy(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 }
 
 // Fills FusedArgs for src/dst if the fused kernel applies; returns false
-// otherwise (the caller then runs the per-axis chain).
+// otherwise (the caller then runs the per-axis chain).  edge: plan for the
+// instances with edge weight rows (the caller fills fa.etab / eL / eR);
+// role_dims (optional, 3 entries) receives the tensor dimension of the x, y
+// and z roles (-1: none).
 bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                 const ndl_axis_kernel* k, int accum, FusedArgs& fa, FusedConfig& cfg, bool& int_acc,
-                int tz_override = 0)
+                int tz_override = 0, bool edge = false, int* role_dims = nullptr)
 {
     if (env_int("NDL_DISABLE_FUSED", 0)) return false;
     if (src->dtype != dst->dtype || naxes > 3) return false;
@@ -387,9 +390,9 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
         batch *= src->extent[d];
         fa.nb++;
     }
-    auto set_axis = [&](int d, int32_t& kk, int32_t& c, int32_t& b, float* wf, int32_t* wi, bool lag) {
+    auto set_axis = [&](int d, int32_t& kk, int32_t& c, int32_t& b, float* wf, bool lag) {
         if (d < 0 || conv_of_dim[d] < 0) {
-            kk = 1; c = 0; b = NDL_CLAMP; wf[0] = 1.0f; wi[0] = 1;
+            kk = 1; c = 0; b = NDL_CLAMP; wf[0] = 1.0f;
             return;
         }
         const ndl_axis_kernel& ak = k[conv_of_dim[d]];
@@ -397,18 +400,19 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
         for (int t = 0; t < ak.count; t++) {
             int idx = lag ? ak.count - 1 - t : t;
             wf[idx] = (float)ak.taps[t];
-            wi[idx] = (int32_t)ak.taps[t];
         }
     };
-    set_axis(xd, fa.kx, fa.cx, fa.bx, fa.wx, fa.iwx, false);
-    set_axis(yd, fa.ky, fa.cy, fa.by, fa.wy, fa.iwy, false);
-    set_axis(zd, fa.kz, fa.cz, fa.bz, fa.wzlag, fa.iwzlag, true);
+    set_axis(xd, fa.kx, fa.cx, fa.bx, fa.wx, false);
+    set_axis(yd, fa.ky, fa.cy, fa.by, fa.wy, false);
+    set_axis(zd, fa.kz, fa.cz, fa.bz, fa.wzlag, true);
     cfg.kmax = 25;
     cfg.tx = 64;
     cfg.ty = 16;
     cfg.cpt = env_int("NDL_FUSED_CPT", 2);
     if (cfg.cpt != 2 && cfg.cpt != 4) cfg.cpt = 2;
     cfg.il = il;
+    cfg.edge = edge ? 1 : 0;
+    if (edge && il != 1) return false;
     cfg.ring = fused_ring_for(fa.kz);
     if (cfg.ring < 0) return false;
     if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) {
@@ -444,6 +448,11 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     fa.fdy.init((uint32_t)fa.tiles_y);
     fa.fdz.init((uint32_t)fa.tiles_z);
     fa.xcd_remap = (ntiles % 8 == 0) && env_int("NDL_XCD_REMAP", 1);
+    if (role_dims) {
+        role_dims[0] = xd;
+        role_dims[1] = yd;
+        role_dims[2] = zd;
+    }
     return true;
 }
 
@@ -480,13 +489,14 @@ int check_forward_args(const ndl_tensor* src, const ndl_tensor* dst, int32_t nax
 // ---------------------------------------------------------------- adjoint
 // Transposed axis kernel: wrap transposes to wrap with flipped taps; reflect
 // with symmetric taps and a centred odd kernel is self-adjoint; clamp and
-// asymmetric reflect need per-edge weight rows (PASS_TABLE).
+// asymmetric reflect also need weight rows for the K rows nearest each end
+// (EdgeTables, built on the device).
 struct AdjAxis {
     std::vector<double> taps; // flipped
     ndl_axis_kernel k;
     bool table;
-    std::vector<double> rows; // (EL+ER) x K
-    int EL = 0, ER = 0;
+    int EL = 0, ER = 0;       // edge rows at the start / end
+    size_t nw = 0;            // (EL + ER) * K weights
 };
 
 bool symmetric(const ndl_axis_kernel& k)
@@ -509,23 +519,11 @@ void build_adjoint_axis(const ndl_axis_kernel& k, int64_t m, AdjAxis& out)
     out.table = !(k.border == NDL_WRAP || (k.border == NDL_REFLECT && symmetric(k)));
     if (!out.table) return;
     // row i of A^T in window form: out[i] = sum_s V[i][s] * g[i + s - c'],
-    // V[i][s] = sum_t w[t] [B(j + t - c) == i], j = i + s - c'.  Rows farther
-    // than K from either edge equal the flipped band.
-    const int cp = K - 1 - c;
+    // V[i][s] = sum_t w[t] [B(j + t - c) == i], j = i + s - c'.  Rows at
+    // least K - 1 from both ends equal the flipped band.
     out.EL = (int)std::min<int64_t>(K, m);
     out.ER = (int)std::min<int64_t>(K, m - out.EL);
-    out.rows.assign((size_t)(out.EL + out.ER) * K, 0.0);
-    for (int r = 0; r < out.EL + out.ER; r++) {
-        int64_t i = r < out.EL ? r : m - out.ER + (r - out.EL);
-        for (int s = 0; s < K; s++) {
-            int64_t j = i + s - cp;
-            double v = 0;
-            if (j >= 0 && j < m)
-                for (int t = 0; t < K; t++)
-                    if (border_map(k.border, m, j + t - c) == i) v += k.taps[t];
-            out.rows[(size_t)r * K + s] = v;
-        }
-    }
+    out.nw = (size_t)(out.EL + out.ER) * K;
 }
 
 int adjoint_input_impl(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes, const int32_t* axes,
@@ -546,32 +544,87 @@ int adjoint_input_impl(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes,
     }
     if (!any_table) return forward_impl(g, gx, naxes, rax.data(), rk.data(), accum, NDL_ROUND_TRUNC, s);
 
+    // edge weight rows of every table axis, built on the device (no host
+    // buffer has to outlive this call)
+    auto fill_tables = [&](const std::vector<int>& which, const std::vector<void*>& dst, int dt) -> int {
+        for (size_t g0 = 0; g0 < which.size(); g0 += 3) {
+            EdgeTables et;
+            memset(&et, 0, sizeof et);
+            for (size_t q = g0; q < which.size() && q < g0 + 3; q++) {
+                const int i = which[q], e = et.naxes++;
+                et.K[e] = k[i].count;
+                et.c[e] = k[i].center;
+                et.border[e] = k[i].border;
+                et.EL[e] = adj[i].EL;
+                et.ER[e] = adj[i].ER;
+                et.M[e] = g->extent[axes[i]];
+                for (int t = 0; t < k[i].count; t++) et.taps[e][t] = k[i].taps[t];
+                if (dt == NDL_F64) et.outd[e] = (double*)dst[q];
+                else et.outf[e] = (float*)dst[q];
+            }
+            hipError_t e = launch_edge_tables(et, s);
+            if (e != hipSuccess) return hip_fail(e, "edge-table build");
+        }
+        return NDL_OK;
+    };
+
+    // one fused pass: the band in the tile's taps, weight rows at the ends
+    if (g->dtype == gx->dtype) {
+        FusedArgs fa;
+        FusedConfig cfg;
+        bool int_acc = false;
+        int rd[3];
+        if (plan_fused(g, gx, naxes, rax.data(), rk.data(), accum, fa, cfg, int_acc, 0, true, rd)) {
+            std::vector<int> which, role;
+            size_t nfl = 0;
+            for (int r = 0; r < 3; r++)
+                for (int i = 0; rd[r] >= 0 && i < naxes; i++)
+                    if (axes[i] == rd[r] && adj[i].table) {
+                        which.push_back(i);
+                        role.push_back(r);
+                        nfl += adj[i].nw;
+                    }
+            Scratch etab;
+            int rc = alloc_scratch(etab, nfl * sizeof(float), s);
+            if (rc) return rc;
+            std::vector<void*> dst;
+            size_t off = 0;
+            for (size_t q = 0; q < which.size(); q++) {
+                float* p = (float*)etab.p + off;
+                dst.push_back(p);
+                fa.etab[role[q]] = p;
+                fa.eL[role[q]] = adj[which[q]].EL;
+                fa.eR[role[q]] = adj[which[q]].ER;
+                off += adj[which[q]].nw;
+            }
+            if ((rc = fill_tables(which, dst, NDL_F32))) return rc;
+            hipError_t e = launch_fused(fa, cfg, g->dtype, int_acc, s);
+            if (e == hipSuccess) return NDL_OK;
+            if (e != hipErrorInvalidValue) return hip_fail(e, "fused adjoint launch");
+        }
+    }
+
     const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
     const size_t acc_sz = dtype_size(acc_dt);
     Scratch buf[2], tabs;
     ndl_tensor mid[2];
     size_t tab_bytes = 0;
-    for (auto& a : adj) tab_bytes += a.rows.size() * acc_sz;
+    for (auto& a : adj) tab_bytes += a.nw * acc_sz;
     int rc = alloc_scratch(tabs, tab_bytes, s);
     if (rc) return rc;
-    std::vector<unsigned char> host_tab(std::max<size_t>(tab_bytes, 1));
     std::vector<size_t> tab_off(naxes, 0);
+    std::vector<int> which;
+    std::vector<void*> dst;
     size_t off = 0;
     for (int i = 0; i < naxes; i++) {
         tab_off[i] = off;
-        for (size_t r = 0; r < adj[i].rows.size(); r++) {
-            if (acc_dt == NDL_F64) ((double*)(host_tab.data() + off))[r] = adj[i].rows[r];
-            else ((float*)(host_tab.data() + off))[r] = (float)adj[i].rows[r];
+        if (adj[i].table) {
+            which.push_back(i);
+            dst.push_back((unsigned char*)tabs.p + off);
         }
-        off += adj[i].rows.size() * acc_sz;
+        off += adj[i].nw * acc_sz;
     }
-    if (tab_bytes) {
-        hipError_t e = hipMemcpyAsync(tabs.p, host_tab.data(), tab_bytes, hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return hip_fail(e, "edge-table upload");
-        // the host buffer dies with this call: make sure the copy has read it
-        e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "edge-table upload");
-    }
+    if ((rc = fill_tables(which, dst, acc_dt))) return rc;
     if (naxes > 1) {
         size_t bytes = (size_t)numel(g) * acc_sz;
         for (int i = 0; i < 2 && i < naxes - 1; i++) {

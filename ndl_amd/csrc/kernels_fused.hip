@@ -1,374 +1,10 @@
-// kernels_fused.hip -- up to three convolved axes in one pass over HBM.
-//
-// A workgroup of TX*TY/CPT threads owns a TX x TY tile of output columns
-// (x = the unit-stride dimension, y = a second axis) and marches along a third
-// axis z, one input slice at a time:
-//   1. the slice's (TY+ky-1) x (TX+kx-1) input window, border indices
-//      resolved once per workgroup (detail.h:178-194 semantics), is requested
-//      two steps ahead into registers (two buffers) and published to LDS;
-//   2. y-pass: LDS -> LDS, 4 rows per task;
-//   3. x-pass: LDS -> registers, CPT consecutive x outputs per thread;
-//   steps are software-pipelined (y-pass of slice s beside the x/z-pass of
-//   slice s-1, double-buffered LDS) so each step needs one barrier;
-//   4. z-pass: each thread keeps the last P = kz x-filtered slices of its
-//      columns in a register ring; once kz slices are in, one output slice is
-//      written.  The slice loop is unrolled by 2P so every ring slot and
-//      prefetch buffer is a compile-time register choice.
-// Every input element is read from HBM once per tile and every output written
-// once; only the y/x halos are re-read (from L2) and re-computed.
-//
-// Tap counts are compile-time for the common filters (everything unrolled,
-// taps in scalar registers, LDS offsets as immediates); a generic
-// instantiation per ring period takes kx/ky at run time (4-tap chunks).
-//
-// Accumulation: FP32 FMA.  Integer (uint8) storage reaches this kernel only
-// with integer taps and sums below 2^24, so every float value is an exact
-// integer and the result is bit-identical to the reference's FP64 sum
-// (convolution.h:64-70).  The host routes filters with an exactly-zero tap
-// to the per-axis chain, which skips them (detail.h:164-174).
-#include <type_traits>
-#include <utility>
-
-#include "ndl_internal.hpp"
+// kernels_fused.hip -- forward instances of the fused separable kernel
+// (fused_tile.hpp) and the launch / occupancy dispatch.
+#include "fused_tile.hpp"
 
 namespace ndl {
 namespace {
-
-template <class TS> __device__ __forceinline__ float load_s(const TS* p)
-{
-    if constexpr (is_int_storage<TS>::value) return (float)*p;
-    else return load_f32(p);
-}
-template <class TS> __device__ __forceinline__ void store_s(TS* p, float v)
-{
-    if constexpr (is_int_storage<TS>::value) *p = (TS)(uint32_t)(int32_t)v; // exact integer, low bits (convolution.h:69)
-    else store_from_f32(p, v);
-}
-
-template <int V> using IC = std::integral_constant<int, V>;
-
-template <class F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
-{
-    (f(IC<I>{}), ...);
-}
-// f(IC<0>{}), ..., f(IC<N-1>{}) in order
-template <int N, class F> __device__ __forceinline__ void static_for(F&& f)
-{
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-// acc[o] += sum_{j<NTAP} w[j] * win[o + j], o < NO, taps ascending
-// (taps IL window elements apart: interleaved channels, IL = channel count)
-template <int NO, int NTAP, int NW, int IL = 1>
-__device__ __forceinline__ void fma_block(float (&acc)[NO], const float (&win)[NW], const float* w)
-{
-    static_assert(NO + (NTAP - 1) * IL <= NW, "window too short");
-#pragma unroll
-    for (int j = 0; j < NTAP; j++) {
-        const float wj = w[j];
-#pragma unroll
-        for (int o = 0; o < NO; o++) acc[o] = __builtin_fmaf(win[o + j * IL], wj, acc[o]);
-    }
-}
-
-// 1-D correlation of NO consecutive outputs read through get(i) (window
-// element i).  KS > 0: KS taps, fully unrolled.  KS == 0: K taps at run time,
-// 4-tap chunks then a 1..3-tap tail.
-template <int NO, int KS, int IL = 1, class Get>
-__device__ __forceinline__ void correlate(float (&acc)[NO], int K, const float* w, Get get)
-{
-    static_assert(KS > 0 || IL == 1, "interleaved taps need a compile-time tap count");
-    if constexpr (KS > 0) {
-        constexpr int NW = NO + (KS - 1) * IL;
-        float win[NW];
-#pragma unroll
-        for (int i = 0; i < NW; i++) win[i] = get(i);
-        fma_block<NO, KS, NW, IL>(acc, win, w);
-    } else {
-        constexpr int NW = NO + 3;
-        int t0 = 0;
-        for (; t0 + 4 <= K; t0 += 4) {
-            float win[NW];
-#pragma unroll
-            for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
-            fma_block<NO, 4, NW>(acc, win, w + t0);
-        }
-        const int rem = K - t0;
-        if (rem > 0) {
-            float win[NW];
-#pragma unroll
-            for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
-            if (rem == 3) fma_block<NO, 3, NW>(acc, win, w + t0);
-            else if (rem == 2) fma_block<NO, 2, NW>(acc, win, w + t0);
-            else fma_block<NO, 1, NW>(acc, win, w + t0);
-        }
-    }
-}
-
-// Ring step U of period P (= kz): this slice's x-filtered values go to slot U;
-// once P slices are in, output z = z0 + s - (P-1) is
-// sum_t w[t] * in[z + t - cz], t ascending = lag l = P-1-t descending.
-template <int U, int P, int CPT, class TS>
-__device__ __forceinline__ void ring_step(float (&ring)[CPT][P], const float (&xv)[CPT], int s, const FusedArgs& a,
-                                          TS* out, bool row_ok, int x_first)
-{
-#pragma unroll
-    for (int o = 0; o < CPT; o++) ring[o][U] = xv[o];
-    if (s < P - 1) return;
-    float acc[CPT];
-#pragma unroll
-    for (int o = 0; o < CPT; o++) acc[o] = 0.f;
-#pragma unroll
-    for (int l = P - 1; l >= 0; l--) {
-        const float w = a.wzlag[l];
-#pragma unroll
-        for (int o = 0; o < CPT; o++) acc[o] = __builtin_fmaf(ring[o][(U - l + P) % P], w, acc[o]);
-    }
-    if (row_ok) {
-#pragma unroll
-        for (int o = 0; o < CPT; o++)
-            if (x_first + o < a.X) store_s<TS>(out + o, acc[o]);
-    }
-}
-
-// One tile: `tile` indexes (x tile, y tile, z chunk, batch) with x fastest.
-// KXS/KYS: compile-time x/y tap counts, 0 = taken from the arguments.
-// LDS (floats): sIn = (NR+3) x NC input window (+3 rows of slack for the
-// chunked y reads), sY = TY x NCP y-filtered rows (NCP >= NC+3, a multiple
-// of 4, slack for the chunked x reads).
-template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL = 1>
-__device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, unsigned char* smem_raw)
-{
-    constexpr int NT = TX * TY / CPT;
-    constexpr int XG = TX / CPT;
-    static_assert(CPT >= 1 && XG * CPT == TX && XG * TY == NT, "tile must map CPT x-outputs per thread");
-    constexpr int RY = 4;
-    static_assert(TY % RY == 0, "TY must be a multiple of 4");
-    constexpr int KMAX = 25;
-    constexpr int NCMAX = TX + ((KXS > 0 ? KXS : KMAX) - 1) * IL;
-    constexpr int NRMAX = TY + (KYS > 0 ? KYS : KMAX) - 1;
-    constexpr int NLMAX = (NCMAX * NRMAX + NT - 1) / NT;
-    constexpr int NTASKMAX = (NCMAX * (TY / RY) + NT - 1) / NT;
-
-    const int kx = KXS > 0 ? KXS : a.kx;
-    const int ky = KYS > 0 ? KYS : a.ky;
-    const int NC = TX + (kx - 1) * IL; // window columns (x taps IL apart)
-    const int NR = TY + ky - 1;
-    const int NCP = (NC + 3 + 3) & ~3;
-    const int NIN = NR * NC;
-    // two input windows and two y-filtered planes (pipelined steps)
-    const int IN_FL = ((NR + 3) * NC + 3) & ~3;
-    const int Y_FL = TY * NCP;
-    float* sIn0 = (float*)smem_raw;
-    float* sIn1 = sIn0 + IN_FL;
-    float* sY0 = sIn1 + IN_FL;
-    float* sY1 = sY0 + Y_FL;
-
-    uint32_t q = a.fdx.div(tile);
-    const int tix = (int)(tile - q * (uint32_t)a.tiles_x);
-    uint32_t q2 = a.fdy.div(q);
-    const int tiy = (int)(q - q2 * (uint32_t)a.tiles_y);
-    uint32_t bi = a.fdz.div(q2);
-    const int tiz = (int)(q2 - bi * (uint32_t)a.tiles_z);
-    int64_t soff = 0, doff = 0;
-    for (int d = 0; d < a.nb; d++) {
-        uint32_t qq = a.bdiv[d].div(bi);
-        int64_t c = (int64_t)(bi - qq * (uint32_t)a.bext[d]);
-        bi = qq;
-        soff += c * a.sb[d];
-        doff += c * a.db[d];
-    }
-    const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * a.tz;
-    const int nz = min(a.tz, a.Z - z0);
-    const int nsl = nz + P - 1;
-    const TS* src = (const TS*)a.src + soff;
-    TS* dst = (TS*)a.dst + doff;
-    const int tid = threadIdx.x;
-
-    // per-thread load slots: element i = tid + k*NT of the window, border
-    // indices resolved once for the whole march
-    int32_t loff[NLMAX];
-#pragma unroll
-    for (int k = 0; k < NLMAX; k++) {
-        const int i = tid + k * NT;
-        loff[k] = 0;
-        if (i < NIN) {
-            const int r = i / NC;
-            const int c = i - r * NC;
-            const int yy = safe_map32(a.by, a.Y, y0 - a.cy + r);
-            int xx;
-            if constexpr (IL == 1) {
-                xx = safe_map32(a.bx, a.X, x0 - a.cx + c);
-            } else {
-                // merged (channel, x) index: map x, keep the channel
-                const int e = x0 - a.cx * IL + c;
-                const int w = e >= 0 ? e / IL : -((-e + IL - 1) / IL);
-                xx = safe_map32(a.bx, a.X / IL, w) * IL + (e - w * IL);
-            }
-            loff[k] = (int32_t)((int64_t)yy * a.sy + xx);
-        }
-    }
-    // y-pass tasks: column c, rows [r0, r0+RY)
-    int tcol[NTASKMAX], trow[NTASKMAX];
-#pragma unroll
-    for (int k = 0; k < NTASKMAX; k++) {
-        const int task = tid + k * NT;
-        const int rg = task / NC;
-        tcol[k] = task - rg * NC;
-        trow[k] = rg * RY;
-    }
-    const int ntask = NC * (TY / RY);
-    // x-pass / ring ownership: CPT consecutive x of row ry
-    const int ry = tid / XG;
-    const int xs = (tid - ry * XG) * CPT;
-    const bool row_ok = y0 + ry < a.Y;
-
-    // the run-time-tap path reads up to 3 rows / columns past a window in
-    // 4-tap chunks: that slack must hold finite values.  Compile-time taps
-    // read exactly the window.
-    if constexpr (KXS == 0 || KYS == 0) {
-        for (int i = tid; i < 3 * NC; i += NT) {
-            sIn0[NIN + i] = 0.f;
-            sIn1[NIN + i] = 0.f;
-        }
-        for (int i = tid; i < TY * (NCP - NC); i += NT) {
-            const int r = i / (NCP - NC), c = NC + i - r * (NCP - NC);
-            sY0[r * NCP + c] = 0.f;
-            sY1[r * NCP + c] = 0.f;
-        }
-    }
-
-    // two register buffers: slice s+1 is in flight while slice s+2 is
-    // requested, so a load has two slice steps to arrive
-    float pre[NLMAX], pre2[NLMAX];
-    // the z border map only matters for chunks that reach past an end
-    const bool z_interior = z0 - a.cz >= 0 && z0 - a.cz + nsl <= a.Z;
-    auto load_slice = [&](int s, float (&dstv)[NLMAX]) {
-        const int zz = z_interior ? z0 - a.cz + s : safe_map32(a.bz, a.Z, z0 - a.cz + s);
-        const TS* base = src + (int64_t)zz * a.sz;
-#pragma unroll
-        for (int k = 0; k < NLMAX; k++)
-            if (tid + k * NT < NIN) dstv[k] = load_s<TS>(base + loff[k]);
-    };
-
-    float ring[CPT][P];
-#pragma unroll
-    for (int o = 0; o < CPT; o++)
-#pragma unroll
-        for (int p = 0; p < P; p++) ring[o][p] = 0.f;
-
-    // y-pass of one slice: window `in` -> y-filtered plane `yp`
-    auto y_pass = [&](const float* in, float* yp) {
-#pragma unroll
-        for (int k = 0; k < NTASKMAX; k++) {
-            if (tid + k * NT < ntask) {
-                const float* col = in + trow[k] * NC + tcol[k];
-                float o[RY];
-#pragma unroll
-                for (int r = 0; r < RY; r++) o[r] = 0.f;
-                correlate<RY, KYS>(o, ky, a.wy, [&](int i) { return col[i * NC]; });
-#pragma unroll
-                for (int r = 0; r < RY; r++) yp[(trow[k] + r) * NCP + tcol[k]] = o[r];
-            }
-        }
-    };
-    // x-pass of one y-filtered plane, then ring slot U (slice s)
-    auto xz_pass = [&](auto slot, int s, const float* yp) {
-        constexpr int U = decltype(slot)::value;
-        float xv[CPT];
-#pragma unroll
-        for (int o = 0; o < CPT; o++) xv[o] = 0.f;
-        const float* row = yp + ry * NCP + xs;
-        correlate<CPT, KXS, IL>(xv, kx, a.wx, [&](int i) { return row[i]; });
-        const int zout = z0 + s - (P - 1);
-        TS* out = dst + (int64_t)zout * a.dz + (int64_t)(y0 + ry) * a.dy + (x0 + xs);
-        ring_step<U, P, CPT, TS>(ring, xv, s, a, out, row_ok, x0 + xs);
-    };
-    auto publish = [&](const float (&v)[NLMAX], float* in) {
-#pragma unroll
-        for (int k = 0; k < NLMAX; k++)
-            if (tid + k * NT < NIN) in[tid + k * NT] = v[k];
-    };
-
-    // Pipelined schedule, one barrier per step.  Step s: publish slice s+1
-    // (its registers then fetch slice s+3), y-pass slice s, x/z-pass slice
-    // s-1.  Slice j lives in registers pre[j&1], LDS window sIn[j&1] and
-    // y-plane sY[j&1]; the barrier ends every read of the buffers the next
-    // step overwrites.
-    load_slice(0, pre);
-    if (nsl > 1) load_slice(1, pre2);
-    __syncthreads(); // slack fill done
-    publish(pre, sIn0);
-    if (nsl > 2) load_slice(2, pre);
-    __syncthreads();
-    auto step = [&](auto ic, int s) {
-        constexpr int u = decltype(ic)::value; // s = s0 + u, s0 a multiple of 2P
-        constexpr bool even = (u % 2) == 0;
-        if (s + 1 < nsl) {
-            if constexpr (even) {
-                publish(pre2, sIn1);
-                if (s + 3 < nsl) load_slice(s + 3, pre2);
-            } else {
-                publish(pre, sIn0);
-                if (s + 3 < nsl) load_slice(s + 3, pre);
-            }
-        }
-        if (s < nsl) y_pass(even ? sIn0 : sIn1, even ? sY0 : sY1);
-        if (s >= 1) xz_pass(IC<(u + P - 1) % P>{}, s - 1, even ? sY1 : sY0);
-        __syncthreads();
-    };
-    // 2P steps per iteration keep ring slots and buffers compile-time
-    for (int s0 = 0; s0 <= nsl; s0 += 2 * P) {
-        static_for<2 * P>([&](auto ic) {
-            const int s = s0 + decltype(ic)::value;
-            if (s <= nsl) step(ic, s);
-        });
-    }
-}
-
-template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL>
-__global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedArgs a)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    // each XCD gets a contiguous range of tiles when remapping (workgroups
-    // are dealt round-robin over the 8 XCDs)
-    uint32_t tile = blockIdx.x;
-    if (a.xcd_remap) {
-        const uint32_t per = a.ntiles >> 3;
-        tile = (tile & 7u) * per + (tile >> 3);
-    }
-    fused_tile<TS, TX, TY, CPT, KXS, KYS, P, IL>(a, tile, smem_raw);
-}
-
-// Persistent work queue over many images with the same z tap count:
-// workgroups take tiles from one counter until all `total` tiles of all jobs
-// are done.  tile_start[j] is the first global tile of job j (prefix sums,
-// tile_start[njobs] = total).
-template <class TS, int TX, int TY, int CPT, int P>
-__global__ __launch_bounds__(TX * TY / CPT) void fused_queue_kernel(const FusedArgs* __restrict__ jobs,
-                                                                    const uint32_t* __restrict__ tile_start,
-                                                                    int njobs, uint32_t total, uint32_t* counter)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ uint32_t next;
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t t = __builtin_amdgcn_readfirstlane(next);
-        __syncthreads();
-        if (t >= total) break; // every workgroup reaches this once the counter passes total
-        int lo = 0, hi = njobs;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (tile_start[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        lo = __builtin_amdgcn_readfirstlane(lo);
-        fused_tile<TS, TX, TY, CPT, 0, 0, P>(jobs[lo], t - tile_start[lo], smem_raw);
-        __syncthreads(); // LDS is reused by the next tile
-    }
-}
+using namespace fused_detail;
 
 __global__ void reset_counter_kernel(uint32_t* c) { *c = 0u; }
 
@@ -440,11 +76,15 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
     const int NC = cfg.tx + (a.kx - 1) * cfg.il;
     const int NR = cfg.ty + a.ky - 1;
     const int NCP = (NC + 3 + 3) & ~3;
-    return (size_t)2 * ((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP) * 4;
+    size_t floats = (size_t)2 * ((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP);
+    // edge instances: weight rows of x, y and z (at most K rows per end)
+    if (cfg.edge) floats += (size_t)2 * (a.kx * a.kx + a.ky * a.ky + a.kz * a.kz);
+    return floats * 4;
 }
 
 bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue)
 {
+    if (cfg.edge) return !queue && fused_edge_supported(cfg, dt, kx, ky);
     if (queue) return with_queue_instance(cfg, dt, [](auto) { return hipSuccess; }) == hipSuccess;
     return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
 }
@@ -454,6 +94,7 @@ hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool
     if (a.ntiles == 0) return hipSuccess;
     if (cfg.ring != a.kz) return hipErrorInvalidValue;
     const size_t lds = fused_lds_bytes(a, cfg, false);
+    if (cfg.edge) return launch_fused_edge(a, cfg, dt, lds, s);
     return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il) {
         using TS = decltype(ts);
         constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
@@ -467,6 +108,7 @@ hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool
 hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int dt, bool, size_t lds, hipStream_t s)
 {
     if (q.total == 0) return hipSuccess;
+    if (cfg.edge) return hipErrorInvalidValue;
     if (!fused_supported(cfg, dt, 0, 0, true)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, q.counter);
     return with_queue_instance(cfg, dt, [&](auto r) {
@@ -479,6 +121,7 @@ hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int d
 
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
+    if (cfg.edge) return fused_edge_blocks_per_cu(cfg, dt, kx, ky, lds);
     int n = 0;
     hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il) {
         using TS = decltype(ts);

@@ -92,12 +92,17 @@ struct FusedArgs {
     FastDiv fdx, fdy, fdz;
     int32_t xcd_remap;            // 1: give each XCD a contiguous range of tiles
     float wx[32], wy[32], wzlag[32];   // float accumulation taps
-    int32_t iwx[32], iwy[32], iwzlag[32]; // integer accumulation taps
+    // edge weight rows per role (x, y, z): eL rows at the start and eR at
+    // the end of the axis, k floats each in window order; null = the band
+    // everywhere.  Only the EDGE instances read them.
+    const float* etab[3];
+    int32_t eL[3], eR[3];
 };
 struct FusedConfig {
     int tx, ty, ring, kmax;       // template selection
     int cpt;                      // x outputs per thread (threads = tx*ty/cpt)
     int il;                       // interleave: x taps il elements apart (channels)
+    int edge;                     // 1: rows near the ends take weight rows (FusedArgs::etab)
 };
 // Returns hipErrorInvalidValue when no instantiation matches cfg/dtype.
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool int_acc,
@@ -114,6 +119,25 @@ struct FusedQueue {
 hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int dt, bool int_acc, size_t lds,
                               hipStream_t s);
 int fused_ring_for(int kz);       // ring period for kz, or -1 when unsupported
+// Instances with edge weight rows (cfg.edge), kernels_fused_edge.hip.
+hipError_t launch_fused_edge(const FusedArgs& a, const FusedConfig& cfg, int dt, size_t lds, hipStream_t s);
+bool fused_edge_supported(const FusedConfig& cfg, int dt, int kx, int ky);
+int fused_edge_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
+
+// Edge weight rows of transposed axis operators, built on the device:
+// for axis a of extent M with taps w (count K, centre c) and border B,
+// row r covers output i = r (r < EL) or M - ER + (r - EL), and
+// V[r][s] = sum_t w[t] [B(j + t - c) == i] with j = i + s - (K-1-c)
+// (0 when j is outside [0, M)); sums in FP64, t ascending.
+struct EdgeTables {
+    int32_t naxes;
+    int32_t K[3], c[3], border[3], EL[3], ER[3];
+    int64_t M[3];
+    double taps[3][NDL_MAX_TAPS];
+    float* outf[3];   // float rows (or null)
+    double* outd[3];  // double rows (or null)
+};
+hipError_t launch_edge_tables(const EdgeTables& e, hipStream_t s);
 // resident workgroups per CU for this instantiation (0 if unknown)
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
 // whether an instantiation exists (queue: the persistent work-queue kernel)

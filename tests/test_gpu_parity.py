@@ -259,6 +259,52 @@ def test_adjoint_input(border, symmetric):
     assert bound_err(out.to_numpy(), want32, g32, taps) <= 1e-6
 
 
+def _edge_case(name):
+    """Adjoint cases whose clamp / asymmetric-reflect axes take the fused
+    kernel with edge weight rows: (numpy shape, ndl axes, taps, borders)."""
+    rng = np.random.default_rng(21)
+    g = workloads.gaussian_taps
+    if name == "rand9_clamp":       # compile-time 9/9/9; 3 x tiles, 10 y tiles
+        return (40, 150, 131), [0, 1, 2], [rng.standard_normal(9) for _ in range(3)], ["clamp"] * 3
+    if name == "rand9_reflect_mixed":  # reflect x, clamp y, wrap z
+        return (33, 47, 200), [0, 1, 2], [rng.standard_normal(9) for _ in range(3)], ["reflect", "clamp", "wrap"]
+    if name == "rand_5_7_11_reflect":  # run-time taps, ring 11
+        return (29, 70, 150), [0, 1, 2], [rng.standard_normal(k) for k in (5, 7, 11)], ["reflect"] * 3
+    if name == "gauss13_clamp_2d":  # compile-time 13/13, ring 1
+        return (300, 257), [0, 1], [g(2.0), g(2.0)], ["clamp"] * 2
+    if name == "video_clamp_wrap":  # compile-time 7/7/5: x, y clamp; t wrap; channel untouched
+        b = np.array([1, 4, 6, 4, 1]) / 16.0
+        return (9, 3, 70, 130), [0, 1, 3], [g(1.0), g(1.0), b], ["clamp", "clamp", "wrap"]
+    if name == "taps_longer_than_axis":  # clamp with 9 taps on a 6-long z axis
+        return (6, 20, 70), [0, 1, 2], [rng.standard_normal(9) for _ in range(3)], ["clamp"] * 3
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["rand9_clamp", "rand9_reflect_mixed", "rand_5_7_11_reflect", "gauss13_clamp_2d",
+                                  "video_clamp_wrap", "taps_longer_than_axis"])
+def test_adjoint_input_fused_edges(name, monkeypatch):
+    shape, axes, taps, borders = _edge_case(name)
+    say("fused adjoint", name, shape)
+    w = workloads.normal(7, 3, int(np.prod(shape))).astype(np.float32).reshape(shape)
+    want = oracle.adjoint_input(w.astype(np.float64), axes, taps, borders)
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    out = ndl_amd.DeviceArray(shape, np.float32)
+    ndl_amd.adjoint_input_dev(dw, out, axes, taps, borders, F32)
+    fused = out.to_numpy()
+    assert bound_err(fused, want, w, taps) <= 1e-6
+    # the per-axis table chain gives the same operator
+    monkeypatch.setenv("NDL_DISABLE_FUSED", "1")
+    out2 = ndl_amd.DeviceArray(shape, np.float32)
+    ndl_amd.adjoint_input_dev(dw, out2, axes, taps, borders, F32)
+    assert bound_err(out2.to_numpy(), want, w, taps) <= 1e-6
+    # and the dot-product identity <A x, w> = <x, A^T w> holds in FP64 terms
+    x = workloads.uniform(8, 0, int(np.prod(shape))).reshape(shape)
+    ax = oracle.sepconv(x, axes, taps, borders, dst=np.zeros(shape))
+    lhs = float(np.sum(ax * w.astype(np.float64)))
+    rhs = float(np.sum(x * fused.astype(np.float64)))
+    assert abs(lhs - rhs) <= 1e-5 * float(np.sum(np.abs(ax * w)))
+
+
 @pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
 def test_adjoint_taps(border):
     rng = np.random.default_rng(19)
