@@ -272,9 +272,9 @@ int device_cus()
 int occupancy(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int, int, int, int, size_t>, int> cache;
+    static std::map<std::tuple<int, int, int, int, int, int, int, int, size_t>, int> cache;
     std::lock_guard<std::mutex> lock(mu);
-    auto key = std::make_tuple(dt, cfg.cpt, cfg.ring, cfg.il, cfg.edge, kx, ky, lds);
+    auto key = std::make_tuple(dt, cfg.ty, cfg.cpt, cfg.ring, cfg.il, cfg.edge, kx, ky, lds);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int b = fused_blocks_per_cu(cfg, dt, kx, ky, lds);
@@ -363,6 +363,22 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     if (zd >= 0 && std::llabs(src->stride[zd]) < std::llabs(src->stride[yd])) std::swap(yd, zd);
     if (yd >= 0) role[yd] = 1;
     if (zd >= 0) role[zd] = 2;
+    // fewer than three convolved axes: untouched dimensions take the free
+    // roles with the identity tap, so tiles are full height (y: the smallest
+    // stride) and march along z (the longest) instead of one workgroup per
+    // row or slice
+    auto free_dim = [&](bool by_stride) {
+        int best = -1;
+        for (int d = 0; d < nd; d++) {
+            if (role[d] != -1 || src->extent[d] <= 1) continue;
+            if (best < 0 || (by_stride ? std::llabs(src->stride[d]) < std::llabs(src->stride[best])
+                                       : src->extent[d] > src->extent[best]))
+                best = d;
+        }
+        return best;
+    };
+    if (yd < 0 && (yd = free_dim(true)) >= 0) role[yd] = 1;
+    if (zd < 0 && (zd = free_dim(false)) >= 0) role[zd] = 2;
     memset(&fa, 0, sizeof fa);
     fa.src = src->data;
     fa.dst = dst->data;
@@ -407,9 +423,9 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     set_axis(zd, fa.kz, fa.cz, fa.bz, fa.wzlag, true);
     cfg.kmax = 25;
     cfg.tx = 64;
-    cfg.ty = 16;
+    cfg.ty = env_int("NDL_FUSED_TY", 16) == 8 ? 8 : 16;
     cfg.cpt = env_int("NDL_FUSED_CPT", 2);
-    if (cfg.cpt != 2 && cfg.cpt != 4) cfg.cpt = 2;
+    if (cfg.cpt != 1 && cfg.cpt != 2 && cfg.cpt != 4) cfg.cpt = 2;
     cfg.il = il;
     cfg.edge = edge ? 1 : 0;
     if (edge && il != 1) return false;
@@ -417,6 +433,7 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     if (cfg.ring < 0) return false;
     if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) {
         cfg.cpt = 2;
+        cfg.ty = 16;
         if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) return false;
     }
     fa.tiles_x = (fa.X + cfg.tx - 1) / cfg.tx;
@@ -657,7 +674,8 @@ int adjoint_taps_impl(const ndl_tensor* x, const ndl_tensor* g, int32_t naxes, c
     int rc = alloc_scratch(ubuf, (size_t)n * acc_sz, s);
     if (rc) return rc;
     ndl_tensor u = dense_like(x, ubuf.p, acc_dt);
-    const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+    // about eight workgroups per CU on large volumes, 2048 voxels per workgroup at least
+    const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>(8 * device_cus(), (n + 2047) / 2048));
     int maxk = 0;
     for (int i = 0; i < naxes; i++) maxk = std::max(maxk, k[i].count);
     if ((rc = alloc_scratch(part, (size_t)nblocks * maxk * sizeof(double), s))) return rc;
@@ -673,27 +691,39 @@ int adjoint_taps_impl(const ndl_tensor* x, const ndl_tensor* g, int32_t naxes, c
             oax.push_back(axes[a]);
             ok.push_back(ndl_axis_kernel{&one, 1, 0, NDL_CLAMP});
         }
-        if ((rc = run_generic_chain(x, &u, (int)oax.size(), oax.data(), ok.data(), accum, NDL_ROUND_TRUNC, s))) return rc;
+        // FP32 accumulation: the fused kernel when it applies (forward_impl
+        // falls back to the chain); exact mode: the FP64 chain
+        rc = accum == NDL_ACCUM_F32
+                 ? forward_impl(x, &u, (int)oax.size(), oax.data(), ok.data(), accum, NDL_ROUND_TRUNC, s)
+                 : run_generic_chain(x, &u, (int)oax.size(), oax.data(), ok.data(), accum, NDL_ROUND_TRUNC, s);
+        if (rc) return rc;
         TapGradArgs ta;
         memset(&ta, 0, sizeof ta);
         ta.g = g->data;
         ta.u = u.data;
-        ta.nd = x->ndim;
-        int order[NDL_MAX_DIM];
-        for (int d = 0; d < x->ndim; d++) order[d] = d;
-        std::stable_sort(order, order + x->ndim,
+        int order[NDL_MAX_DIM], no = 0;
+        for (int d = 0; d < x->ndim; d++)
+            if (d != axes[a]) order[no++] = d;
+        std::stable_sort(order, order + no,
                          [&](int p, int q) { return std::llabs(g->stride[p]) < std::llabs(g->stride[q]); });
-        for (int i = 0; i < x->ndim; i++) {
+        ta.nl = no;
+        int64_t lines = 1;
+        for (int i = 0; i < no; i++) {
             int d = order[i];
             ta.ext[i] = x->extent[d];
             ta.gst[i] = g->stride[d];
             ta.ust[i] = u.stride[d];
             ta.fdiv[i].init((uint32_t)x->extent[d]);
-            if (d == axes[a]) ta.axis_pos = i;
+            lines *= x->extent[d];
         }
-        ta.total = (uint32_t)n;
         ta.m = x->extent[axes[a]];
+        ta.g_as = g->stride[axes[a]];
         ta.u_as = u.stride[axes[a]];
+        const int64_t items = lines * ((ta.m + kTapSeg - 1) / kTapSeg);
+        if (items > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "too many tap-gradient work items");
+        ta.lines = (uint32_t)lines;
+        ta.items = (uint32_t)items;
+        ta.ldiv.init((uint32_t)lines);
         ta.K = k[a].count;
         ta.center = k[a].center;
         ta.border = k[a].border;

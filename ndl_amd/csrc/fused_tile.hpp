@@ -128,7 +128,7 @@ __device__ __forceinline__ int edge_row(int i, int M, int eL, int eR)
 // order t, in place of the band.
 template <int U, int P, int CPT, bool EDGE, class TS>
 __device__ __forceinline__ void ring_step(float (&ring)[CPT][P], const float (&xv)[CPT], int s, const FusedArgs& a,
-                                          TS* out, bool row_ok, int x_first, const float* wedge)
+                                          TS* out, bool row_ok, int x_first, const float* wedge, bool full)
 {
 #pragma unroll
     for (int o = 0; o < CPT; o++) ring[o][U] = xv[o];
@@ -153,7 +153,10 @@ __device__ __forceinline__ void ring_step(float (&ring)[CPT][P], const float (&x
             for (int o = 0; o < CPT; o++) acc[o] = __builtin_fmaf(ring[o][(U - l + P) % P], w, acc[o]);
         }
     }
-    if (row_ok) {
+    if (full) { // uniform: the whole tile is inside x and y
+#pragma unroll
+        for (int o = 0; o < CPT; o++) store_s<TS>(out + o, acc[o]);
+    } else if (row_ok) {
 #pragma unroll
         for (int o = 0; o < CPT; o++)
             if (x_first + o < a.X) store_s<TS>(out + o, acc[o]);
@@ -200,6 +203,16 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     // 2*ky*ky and 2*P*P floats for x, y, z (at most K rows at each end)
     float* sE = sY1 + Y_FL;
     const int eoff_y = 2 * kx * kx, eoff_z = eoff_y + 2 * ky * ky;
+    // run-time taps live in LDS (x at [0, 32), y at [32, 64)): indexing the
+    // by-value argument block at run-time offsets makes the compiler copy it
+    // to scratch.  Compile-time taps are read from the arguments directly.
+    float* sW = sE + (EDGE ? eoff_z + 2 * P * P : 0);
+    const float* wxp;
+    const float* wyp;
+    if constexpr (KXS > 0) wxp = a.wx;
+    else wxp = sW;
+    if constexpr (KYS > 0) wyp = a.wy;
+    else wyp = sW + 32;
 
     uint32_t q = a.fdx.div(tile);
     const int tix = (int)(tile - q * (uint32_t)a.tiles_x);
@@ -259,6 +272,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     const int ry = tid / XG;
     const int xs = (tid - ry * XG) * CPT;
     const bool row_ok = y0 + ry < a.Y;
+    const bool full = x0 + TX <= a.X && y0 + TY <= a.Y;
     // tiles that reach into an edge band (uniform)
     const bool ex = EDGE && a.etab[0] && (x0 < a.eL[0] || x0 + TX > a.X - a.eR[0]);
     const bool ey = EDGE && a.etab[1] && (y0 < a.eL[1] || y0 + TY > a.Y - a.eR[1]);
@@ -267,6 +281,13 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     // 4-tap chunks: that slack must hold finite values.  Compile-time taps
     // read exactly the window.
     if constexpr (KXS == 0 || KYS == 0) {
+        if (tid == 0) {
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                sW[j] = a.wx[j];
+                sW[32 + j] = a.wy[j];
+            }
+        }
         for (int i = tid; i < 3 * NC; i += NT) {
             sIn0[NIN + i] = 0.f;
             sIn1[NIN + i] = 0.f;
@@ -306,7 +327,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
                 float o[RY];
 #pragma unroll
                 for (int r = 0; r < RY; r++) o[r] = 0.f;
-                correlate<RY, KYS>(o, ky, a.wy, [&](int i) NDL_INLINE { return col[i * NC]; });
+                correlate<RY, KYS>(o, ky, wyp, [&](int i) NDL_INLINE { return col[i * NC]; });
                 if constexpr (EDGE) {
                     if (ey) {
 #pragma unroll
@@ -334,7 +355,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
 #pragma unroll
         for (int o = 0; o < CPT; o++) xv[o] = 0.f;
         const float* row = yp + ry * NCP + xs;
-        correlate<CPT, KXS, IL>(xv, kx, a.wx, [&](int i) NDL_INLINE { return row[i]; });
+        correlate<CPT, KXS, IL>(xv, kx, wxp, [&](int i) NDL_INLINE { return row[i]; });
         const int zout = z0 + s - (P - 1);
         const float* wedge = nullptr;
         if constexpr (EDGE) {
@@ -357,7 +378,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
             }
         }
         TS* out = dst + (int64_t)zout * a.dz + (int64_t)(y0 + ry) * a.dy + (x0 + xs);
-        ring_step<U, P, CPT, EDGE, TS>(ring, xv, s, a, out, row_ok, x0 + xs, wedge);
+        ring_step<U, P, CPT, EDGE, TS>(ring, xv, s, a, out, row_ok, x0 + xs, wedge, full);
     };
     auto publish = [&](const float (&v)[NLMAX], float* in) NDL_INLINE {
 #pragma unroll

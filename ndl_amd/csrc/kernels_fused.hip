@@ -12,14 +12,24 @@ __global__ void reset_counter_kernel(uint32_t* c) { *c = 0u; }
 constexpr int kTX = 64, kTY = 16;
 
 // Instantiation table.  Specialised (compile-time taps) filters first, then
-// generic ring periods per storage type.  f(ts, cpt, kx, ky, ring) launches.
+// generic ring periods per storage type.  f(ts, cpt, kx, ky, ring, il, ty)
+// launches.
 template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int kx, int ky, F&& f)
 {
-    if (cfg.tx != kTX || cfg.ty != kTY) return hipErrorInvalidValue;
+    if (cfg.tx != kTX) return hipErrorInvalidValue;
     const int cpt = cfg.cpt, kz = cfg.ring, il = cfg.il;
+    if (cfg.ty == 8) {
+        // 64 x 8 tiles: twice the tiles per slice, so longer z chunks fill the GPU
+        if (dt == NDL_F32 && kx == 11 && ky == 11 && kz == 19 && il == 1) {
+            if (cpt == 2) return f(float{}, IC<2>{}, IC<11>{}, IC<11>{}, IC<19>{}, IC<1>{}, IC<8>{});
+            if (cpt == 1) return f(float{}, IC<1>{}, IC<11>{}, IC<11>{}, IC<19>{}, IC<1>{}, IC<8>{});
+        }
+        return hipErrorInvalidValue;
+    }
+    if (cfg.ty != kTY) return hipErrorInvalidValue;
 #define NDL_SPEC_IL(DT, T, C, KX, KY, KZ, IL)                                                 \
     if (dt == DT && cpt == C && kx == KX && ky == KY && kz == KZ && il == IL)                 \
-        return f(T{}, IC<C>{}, IC<KX>{}, IC<KY>{}, IC<KZ>{}, IC<IL>{});
+        return f(T{}, IC<C>{}, IC<KX>{}, IC<KY>{}, IC<KZ>{}, IC<IL>{}, IC<kTY>{});
 #define NDL_SPEC(DT, T, C, KX, KY, KZ) NDL_SPEC_IL(DT, T, C, KX, KY, KZ, 1)
     // 3-D Gaussian sigma (1.5, 1.5, 3.0): 11/11/19 taps
     NDL_SPEC(NDL_F32, float, 2, 11, 11, 19) NDL_SPEC(NDL_F32, float, 4, 11, 11, 19)
@@ -36,7 +46,7 @@ template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int 
 #undef NDL_SPEC_IL
     if (cpt != 2 || il != 1) return hipErrorInvalidValue;
 #define NDL_GEN(DT, T, KZ) \
-    if (dt == DT && kz == KZ) return f(T{}, IC<2>{}, IC<0>{}, IC<0>{}, IC<KZ>{}, IC<1>{});
+    if (dt == DT && kz == KZ) return f(T{}, IC<2>{}, IC<0>{}, IC<0>{}, IC<KZ>{}, IC<1>{}, IC<kTY>{});
     NDL_GEN(NDL_F32, float, 1) NDL_GEN(NDL_F32, float, 3) NDL_GEN(NDL_F32, float, 5) NDL_GEN(NDL_F32, float, 7)
     NDL_GEN(NDL_F32, float, 9) NDL_GEN(NDL_F32, float, 11) NDL_GEN(NDL_F32, float, 13) NDL_GEN(NDL_F32, float, 15)
     NDL_GEN(NDL_F32, float, 17) NDL_GEN(NDL_F32, float, 19) NDL_GEN(NDL_F32, float, 21) NDL_GEN(NDL_F32, float, 23)
@@ -79,6 +89,7 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
     size_t floats = (size_t)2 * ((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP);
     // edge instances: weight rows of x, y and z (at most K rows per end)
     if (cfg.edge) floats += (size_t)2 * (a.kx * a.kx + a.ky * a.ky + a.kz * a.kz);
+    floats += 64; // run-time x / y taps
     return floats * 4;
 }
 
@@ -86,7 +97,8 @@ bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue)
 {
     if (cfg.edge) return !queue && fused_edge_supported(cfg, dt, kx, ky);
     if (queue) return with_queue_instance(cfg, dt, [](auto) { return hipSuccess; }) == hipSuccess;
-    return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
+    return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto, auto, auto) { return hipSuccess; }) ==
+           hipSuccess;
 }
 
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool, hipStream_t s)
@@ -95,10 +107,10 @@ hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool
     if (cfg.ring != a.kz) return hipErrorInvalidValue;
     const size_t lds = fused_lds_bytes(a, cfg, false);
     if (cfg.edge) return launch_fused_edge(a, cfg, dt, lds, s);
-    return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il) {
+    return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il, auto ty) {
         using TS = decltype(ts);
-        constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
-        hipLaunchKernelGGL((fused_tile_kernel<TS, kTX, kTY, CPT, decltype(kxs)::value, decltype(kys)::value,
+        constexpr int TY = decltype(ty)::value, CPT = decltype(c)::value, NT = kTX * TY / CPT;
+        hipLaunchKernelGGL((fused_tile_kernel<TS, kTX, TY, CPT, decltype(kxs)::value, decltype(kys)::value,
                                               decltype(r)::value, decltype(il)::value>),
                            dim3(a.ntiles), dim3(NT), lds, s, a);
         return hipGetLastError();
@@ -123,12 +135,12 @@ int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t l
 {
     if (cfg.edge) return fused_edge_blocks_per_cu(cfg, dt, kx, ky, lds);
     int n = 0;
-    hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il) {
+    hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il, auto ty) {
         using TS = decltype(ts);
-        constexpr int CPT = decltype(c)::value, NT = kTX * kTY / CPT;
+        constexpr int TY = decltype(ty)::value, CPT = decltype(c)::value, NT = kTX * TY / CPT;
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &n,
-            reinterpret_cast<const void*>(&fused_tile_kernel<TS, kTX, kTY, CPT, decltype(kxs)::value,
+            reinterpret_cast<const void*>(&fused_tile_kernel<TS, kTX, TY, CPT, decltype(kxs)::value,
                                                              decltype(kys)::value, decltype(r)::value,
                                                              decltype(il)::value>),
             NT, lds);

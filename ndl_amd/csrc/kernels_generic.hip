@@ -7,6 +7,8 @@
 // (detail.h:178-194).  NDL_ACCUM_REF accumulates double(src)*double(tap) with
 // every product and sum rounded separately (this file is built with
 // -ffp-contract=off), so results are bit-identical to the reference.
+#include <type_traits>
+
 #include "ndl_internal.hpp"
 
 namespace ndl {
@@ -157,56 +159,115 @@ hipError_t launch_convolve_nd(const NdArgs& a, int in_dt, int out_dt, hipStream_
 // ------------------------------------------------------ tap gradients
 // blockIdx.y = tap.  Each block walks a fixed, strided share of positions,
 // reduces in FP64 through LDS in a fixed tree, so the sum is deterministic.
-template <class TG, class TU>
+// Tap gradient of one axis: grad[t] = sum_p g[p] * U[p with its coordinate
+// j along the axis replaced by B(j + t - c)].  Work item = (segment of
+// kTapSeg consecutive positions along the axis, line); lanes take
+// consecutive lines (coalesced along the fastest other dimension) and share
+// the segment, so the border test is uniform.  A thread loads the U window
+// of its segment once (kTapSeg + K - 1 values) and its kTapSeg g values, and
+// keeps KB >= K FP64 accumulators in registers (taps t >= K are skipped by a
+// uniform test).  g and U convert exactly; one partial per (workgroup, tap),
+// reduced by wave shuffles then across the four waves in LDS;
+// tapgrad_final_kernel sums the partials in a fixed tree order, so the result
+// does not depend on scheduling.
+
+template <class TG, class TU, int KB>
 __global__ __launch_bounds__(256) void tapgrad_partial_kernel(const TapGradArgs a)
 {
-    __shared__ double red[256];
-    const int t = blockIdx.y;
-    double acc = 0;
-    const uint32_t step = 256u * (uint32_t)a.nblocks;
-    for (uint32_t idx = blockIdx.x * 256u + threadIdx.x; idx < a.total; idx += step) {
-        int64_t goff = 0, uoff = 0, j = 0;
-        uint32_t rem = idx;
-        for (int d = 0; d < a.nd; d++) {
+    constexpr int NW = kTapSeg + KB - 1;
+    double acc[KB];
+#pragma unroll
+    for (int t = 0; t < KB; t++) acc[t] = 0.0;
+    const uint32_t step = 256u * gridDim.x;
+    for (uint32_t item = blockIdx.x * 256u + threadIdx.x; item < a.items; item += step) {
+        const uint32_t seg = a.ldiv.div(item);
+        uint32_t rem = item - seg * a.lines;
+        int64_t goff = 0, uoff = 0;
+        for (int d = 0; d < a.nl; d++) {
             uint32_t q = a.fdiv[d].div(rem);
             int64_t c = (int64_t)(rem - q * (uint32_t)a.ext[d]);
             rem = q;
             goff += c * a.gst[d];
             uoff += c * a.ust[d];
-            if (d == a.axis_pos) j = c;
         }
-        int64_t x = j + t - a.center;
-        if (x < 0 || x >= a.m) x = border_map(a.border, a.m, x);
-        double gv = load_f64((const TG*)a.g + goff);
-        double uv = load_f64((const TU*)a.u + (uoff - j * a.u_as + x * a.u_as));
-        acc = acc + gv * uv;
+        const int64_t j0 = (int64_t)seg * kTapSeg;
+        const TU* ub = (const TU*)a.u + uoff;
+        const TG* gb = (const TG*)a.g + goff;
+        // window element i is U at j0 + i - c, border-mapped near the ends;
+        // positions past m - 1 + (K-1-c) are read only by outputs past the
+        // end of the axis (never accumulated) and are clamped to stay in
+        // bounds
+        using W = std::conditional_t<sizeof(TU) == 8, double, float>;
+        W win[NW];
+        const bool inside = j0 - a.center >= 0 && j0 + NW - 1 - a.center < a.m;
+        const int64_t last = a.m - 1 + (a.K - 1 - a.center);
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            win[i] = W(0);
+            if (i < kTapSeg + a.K - 1) {
+                int64_t x = j0 + i - a.center;
+                if (!inside) x = x > last ? a.m - 1 : border_map(a.border, a.m, x);
+                win[i] = ub[x * a.u_as];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kTapSeg; r++) {
+            if (j0 + r < a.m) {
+                const double gv = load_f64(gb + (j0 + r) * a.g_as);
+#pragma unroll
+                for (int t = 0; t < KB; t++)
+                    if (t < a.K) acc[t] = acc[t] + gv * (double)win[r + t];
+            }
+        }
     }
-    red[threadIdx.x] = acc;
+    __shared__ double red[4][KB];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < KB; t++) {
+        if (t < a.K) {
+            double v = acc[t];
+            for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+            if (lane == 0) red[wv][t] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < a.K) {
+        const int t = threadIdx.x;
+        a.partial[(size_t)blockIdx.x * a.K + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+    }
+}
+
+// one workgroup per tap: fixed-order tree over the workgroup partials
+__global__ __launch_bounds__(256) void tapgrad_final_kernel(const double* partial, int nblocks, int K, double* out)
+{
+    __shared__ double red[256];
+    const int t = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) s = s + partial[(size_t)b * K + t];
+    red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.partial[(size_t)blockIdx.x * a.K + t] = red[0];
-}
-
-__global__ void tapgrad_final_kernel(const double* partial, int nblocks, int K, double* out)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= K) return;
-    double s = 0;
-    for (int b = 0; b < nblocks; b++) s = s + partial[(size_t)b * K + t];
-    out[t] = s;
+    if (threadIdx.x == 0) out[t] = red[0];
 }
 
 hipError_t launch_tapgrad_partial(const TapGradArgs& a, int g_dt, int u_is_f64, hipStream_t s)
 {
+    if (a.K < 1 || a.K > 64) return hipErrorInvalidValue;
     auto go = [&](auto tu) {
         using TU = decltype(tu);
         return with_dtype(g_dt, [&](auto tg) {
             using TG = decltype(tg);
-            hipLaunchKernelGGL((tapgrad_partial_kernel<TG, TU>), dim3(a.nblocks, a.K), dim3(256), 0, s, a);
-            return hipGetLastError();
+#define NDL_TG(KB)                                                                                          \
+    if (a.K <= KB) {                                                                                        \
+        hipLaunchKernelGGL((tapgrad_partial_kernel<TG, TU, KB>), dim3(a.nblocks), dim3(256), 0, s, a);       \
+        return hipGetLastError();                                                                           \
+    }
+            NDL_TG(4) NDL_TG(8) NDL_TG(16) NDL_TG(32) NDL_TG(64)
+#undef NDL_TG
+            return hipErrorInvalidValue;
         });
     };
     return u_is_f64 ? go(double{}) : go(float{});
@@ -214,7 +275,7 @@ hipError_t launch_tapgrad_partial(const TapGradArgs& a, int g_dt, int u_is_f64, 
 
 hipError_t launch_tapgrad_final(const double* partial, int nblocks, int K, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(tapgrad_final_kernel, dim3((K + 63) / 64), dim3(64), 0, s, partial, nblocks, K, out);
+    hipLaunchKernelGGL(tapgrad_final_kernel, dim3(K), dim3(256), 0, s, partial, nblocks, K, out);
     return hipGetLastError();
 }
 

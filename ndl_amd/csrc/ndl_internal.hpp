@@ -56,16 +56,20 @@ hipError_t launch_convolve_nd(const NdArgs& a, int in_dt, int out_dt, hipStream_
 // ------------------------------------------- tap-gradient partial sums
 // partial[blk*K + t] = sum over the block's share of positions p of
 // g[p] * u[B(p_axis + t - c)] (FP64), later summed in block order.
+constexpr int kTapSeg = 8;     // tap-gradient positions per work item along the axis
 struct TapGradArgs {
     const void* g;      // grad_out, strided
     const void* u;      // dense intermediate (Acc type)
-    int32_t nd, axis_pos;
+    // lines along the axis: the other dimensions, fastest (by g stride) first
+    int32_t nl;                    // number of other dimensions
     int64_t ext[NDL_MAX_DIM];
     int64_t gst[NDL_MAX_DIM];
     int64_t ust[NDL_MAX_DIM];
     FastDiv fdiv[NDL_MAX_DIM];
-    uint32_t total;
-    int64_t m, u_as;
+    uint32_t lines;                // product of the other extents
+    uint32_t items;                // lines * segments
+    FastDiv ldiv;                  // item -> (segment, line)
+    int64_t m, g_as, u_as;         // axis extent and strides
     int32_t K, center, border;
     double* partial;    // [nblocks][K]
     int32_t nblocks;

@@ -320,6 +320,28 @@ def test_adjoint_taps(border):
     np.testing.assert_allclose(out.to_numpy(), want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
 
 
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+def test_adjoint_taps_f32(border):
+    # FP32 path: the partial results U come from the fused kernel, the
+    # reduction runs in FP64; error bound relative to sum |g * U| (the same
+    # reduction over |x|, |g| and |taps|)
+    rng = np.random.default_rng(22)
+    shape = (40, 70, 131)
+    taps = [rng.standard_normal(k) for k in (3, 5, 9)]
+    x = rng.random(shape).astype(np.float32)
+    w = rng.standard_normal(shape).astype(np.float32)
+    say("adjoint taps f32", border)
+    want = oracle.adjoint_taps(x.astype(np.float64), w.astype(np.float64), [0, 1, 2], taps, [border] * 3)
+    bound = oracle.adjoint_taps(np.abs(x.astype(np.float64)), np.abs(w.astype(np.float64)), [0, 1, 2],
+                                [np.abs(t) for t in taps], [border] * 3)
+    dx = ndl_amd.DeviceArray.from_numpy(x)
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    out = ndl_amd.DeviceArray((17,), np.float64)
+    ndl_amd.adjoint_taps_dev(dx, dw, [0, 1, 2], taps, [border] * 3, out, F32)
+    got = out.to_numpy()
+    assert np.all(np.abs(got - want) <= 2e-6 * bound)
+
+
 # ------------------------------------------------------------------ batched
 def test_plan_matches_single_calls():
     rng = np.random.default_rng(20)
