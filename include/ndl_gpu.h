@@ -72,6 +72,12 @@ enum ndl_rounding {
     NDL_ROUND_NEAREST = 1  /* std::round then cast: gaussian_blur() DIM>=2, :184 */
 };
 
+/* morphology operation (processing/morphology.h:57-101) */
+enum ndl_morph_op {
+    NDL_ERODE = 0,  /* erode(): minimum over the structuring element */
+    NDL_DILATE = 1  /* dilate(): maximum over the structuring element */
+};
+
 enum ndl_status {
     NDL_OK = 0,
     NDL_ERR_ARG = -1,      /* null pointer, bad count, unsupported combination */
@@ -121,6 +127,27 @@ int ndl_sepconv_forward(const ndl_tensor* src, const ndl_tensor* dst, int32_t na
  * kernel: host pointer to prod(kextent) doubles, dense, dimension 0 fastest. */
 int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel,
                     const int64_t* kextent, int32_t border, void* stream);
+
+/* ---- morphology -----------------------------------------------------------
+ * erode(src, dst, kernel, border) / dilate(...), morphology.h:57-101: for each
+ * output, the first included tap (kernel coordinate order, dimension 0
+ * fastest; included = nonzero, detail.h:164-174) and then every strictly
+ * smaller (erode) / larger (dilate) value, border resolved per tap
+ * (detail.h:178-194); dst gets static_cast<DstT> of the winner (SrcT{} when
+ * no tap is included).  Results are bit-identical to the reference.
+ *
+ * ndl_morph_separable: structuring elements whose included set is a product
+ * of per-axis sets (make_box_kernel, image/kernels.h:22-23): k[i].taps is the
+ * inclusion mask of axis axes[i] (nonzero = included), each with its own
+ * border and centre; axes not listed have extent 1.  One min/max pass per
+ * axis.  Same checks as ndl_sepconv_forward; an axis may be listed once. */
+int ndl_morph_separable(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes,
+                        const int32_t* axes, const ndl_axis_kernel* k, int32_t op, void* stream);
+
+/* ndl_morph_nd: any N-D mask (e.g. make_cross_kernel, image/kernels.h:44-58),
+ * one pass; mask as ndl_convolve_nd's kernel. */
+int ndl_morph_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* mask,
+                 const int64_t* kextent, int32_t border, int32_t op, void* stream);
 
 /* ---- adjoint (no counterpart in the reference) -------------------------
  * Gradient of L = <w, sepconv_forward(x)> with respect to x: grad_in =

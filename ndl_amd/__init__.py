@@ -38,6 +38,8 @@ _VP = C.c_void_p
 SIGNATURES = {
     "ndl_sepconv_forward": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, C.c_int32, _VP]),
     "ndl_convolve_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, _VP]),
+    "ndl_morph_separable": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
+    "ndl_morph_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_input": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_taps": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, _VP, C.c_int32, _VP]),
     "ndl_gaussian_taps": (C.c_int, [C.c_double, _DP, C.c_int32]),
@@ -215,6 +217,39 @@ def convolve_nd_dev(src, dst, kernel, border="clamp", stream=None):
     return check(lib().ndl_convolve_nd(C.byref(ts), C.byref(td), kernel.ctypes.data_as(_DP), kext, b, stream))
 
 
+MORPH_OPS = {"erode": 0, "dilate": 1}
+
+
+def morph_separable_dev(src, dst, axes, masks, borders, op, centers=None, stream=None):
+    """erode/dilate with a structuring element that is the product of the
+    per-axis inclusion masks `masks` (nonzero = included)."""
+    ks = _kern(masks, borders, centers)
+    ts, td = src.tensor(), dst.tensor()
+    return check(lib().ndl_morph_separable(C.byref(ts), C.byref(td), len(axes), abi.Kernels.axes(axes), ks.arr,
+                                           MORPH_OPS[op], stream))
+
+
+def morph_nd_dev(src, dst, mask, border, op, stream=None):
+    """erode/dilate with an N-D mask (numpy order, nonzero = included)."""
+    mask = np.ascontiguousarray(mask, dtype=np.float64)
+    kext = (C.c_int64 * abi.NDL_MAX_DIM)(*reversed(mask.shape))
+    b = abi.BORDERS[border] if isinstance(border, str) else int(border)
+    ts, td = src.tensor(), dst.tensor()
+    return check(lib().ndl_morph_nd(C.byref(ts), C.byref(td), mask.ctypes.data_as(_DP), kext, b, MORPH_OPS[op],
+                                    stream))
+
+
+def separable_mask(mask):
+    """Per-axis inclusion masks (numpy order) when the nonzero set of `mask`
+    is their product, else None."""
+    inc = np.asarray(mask) != 0
+    axes = [inc.any(axis=tuple(j for j in range(inc.ndim) if j != i)) for i in range(inc.ndim)]
+    prod = axes[0]
+    for a in axes[1:]:
+        prod = np.multiply.outer(prod, a)
+    return axes if np.array_equal(prod, inc) else None
+
+
 def gaussian_taps(sigma):
     buf = np.zeros(4 * abi.NDL_MAX_TAPS, dtype=np.float64)
     n = check(lib().ndl_gaussian_taps(float(sigma), buf.ctypes.data_as(_DP), len(buf)))
@@ -246,6 +281,26 @@ def convolve(img, kernel, border="clamp", out_dtype=None):
     s = DeviceArray.from_numpy(img)
     d = DeviceArray(img.shape, out_dtype or img.dtype)
     convolve_nd_dev(s, d, kernel, border)
+    return d.to_numpy()
+
+
+def morph(img, mask, op, border="clamp", out_dtype=None, force_nd=False):
+    """ndl::erode / ndl::dilate (morphology.h:57-101) with an N-D mask (numpy
+    order): one min/max pass per axis when the mask is separable (a box),
+    otherwise the one-pass N-D kernel."""
+    s = DeviceArray.from_numpy(img)
+    d = DeviceArray(img.shape, out_dtype or img.dtype)
+    mask = np.asarray(mask, dtype=np.float64)
+    sep = None if force_nd else separable_mask(mask)
+    if sep is not None:
+        n = img.ndim
+        axes = [n - 1 - i for i in range(n) if mask.shape[i] > 1]
+        masks = [sep[i].astype(np.float64) for i in range(n) if mask.shape[i] > 1]
+        if not axes:  # a 1x..x1 element
+            axes, masks = [0], [sep[n - 1].astype(np.float64)]
+        morph_separable_dev(s, d, axes, masks, [border] * len(axes), op)
+    else:
+        morph_nd_dev(s, d, mask, border, op)
     return d.to_numpy()
 
 

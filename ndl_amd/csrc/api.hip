@@ -719,11 +719,19 @@ int adjoint_taps_impl(const ndl_tensor* x, const ndl_tensor* g, int32_t naxes, c
         ta.m = x->extent[axes[a]];
         ta.g_as = g->stride[axes[a]];
         ta.u_as = u.stride[axes[a]];
-        const int64_t items = lines * ((ta.m + kTapSeg - 1) / kTapSeg);
+        // segments along the axis; lanes along lines, or along the segments
+        // of one line when the axis is g's fastest dimension
+        bool fastest = true;
+        for (int d = 0; d < x->ndim; d++)
+            if (d != axes[a] && x->extent[d] > 1 && std::llabs(g->stride[d]) < std::llabs(ta.g_as)) fastest = false;
+        const int64_t nseg = (ta.m + kTapSeg - 1) / kTapSeg;
+        const int64_t items = lines * nseg;
         if (items > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "too many tap-gradient work items");
+        ta.seg = fastest ? 0 : 1;
+        ta.nseg = (uint32_t)nseg;
         ta.lines = (uint32_t)lines;
         ta.items = (uint32_t)items;
-        ta.ldiv.init((uint32_t)lines);
+        ta.ldiv.init(fastest ? (uint32_t)nseg : (uint32_t)lines);
         ta.K = k[a].count;
         ta.center = k[a].center;
         ta.border = k[a].border;
@@ -751,8 +759,11 @@ NDL_API int ndl_sepconv_forward(const ndl_tensor* src, const ndl_tensor* dst, in
     return forward_impl(src, dst, naxes, axes, k, accum, rounding, (hipStream_t)stream);
 }
 
-NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel,
-                            const int64_t* kextent, int32_t border, void* stream)
+// Validates an N-D kernel call and fills NdArgs with the included taps
+// (detail::kernelIncludedTaps: kernel coordinates dimension 0 fastest,
+// nonzero entries only, offsets relative to the centre) uploaded into tb.
+int prepare_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel, const int64_t* kextent,
+               int32_t border, Scratch& tb, NdArgs& a, hipStream_t s)
 {
     int rc;
     if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
@@ -769,11 +780,10 @@ NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const 
             return fail(NDL_ERR_RADIUS, "kernel reaches past one extent on dimension %d", d);
         kn *= kextent[d];
     }
+    memset(&a, 0, sizeof a);
     int64_t n = numel(src);
     if (n == 0) return NDL_OK;
     if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
-    // detail::kernelIncludedTaps: kernel coordinates dimension 0 fastest,
-    // nonzero taps only, offsets relative to the centre
     std::vector<int32_t> offs;
     std::vector<double> vals;
     for (int64_t lin = 0; lin < kn; lin++) {
@@ -785,20 +795,19 @@ NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const 
         }
         vals.push_back(kernel[lin]);
     }
-    hipStream_t s = (hipStream_t)stream;
-    Scratch tb;
     size_t ob = offs.size() * sizeof(int32_t), vb = vals.size() * sizeof(double);
     if ((rc = alloc_scratch(tb, ob + vb + 16, s))) return rc;
     char* dev = (char*)tb.p;
     size_t voff = (ob + 15) & ~(size_t)15;
     std::vector<char> host(voff + vb + 1);
-    memcpy(host.data(), offs.data(), ob);
-    memcpy(host.data() + voff, vals.data(), vb);
-    hipError_t e = hipMemcpyAsync(dev, host.data(), voff + vb, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(e, "kernel tap upload");
-    NdArgs a;
-    memset(&a, 0, sizeof a);
+    if (ob) memcpy(host.data(), offs.data(), ob);
+    if (vb) memcpy(host.data() + voff, vals.data(), vb);
+    if (voff + vb) {
+        hipError_t e = hipMemcpyAsync(dev, host.data(), voff + vb, hipMemcpyHostToDevice, s);
+        // the host copy dies with this call: make sure the transfer has read it
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "kernel tap upload");
+    }
     a.in = src->data;
     a.out = dst->data;
     a.nd = nd;
@@ -813,8 +822,140 @@ NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const 
     a.ntaps = (int32_t)vals.size();
     a.tap_off = (const int32_t*)dev;
     a.tap_val = (const double*)(dev + voff);
-    e = launch_convolve_nd(a, src->dtype, dst->dtype, s);
+    return NDL_OK;
+}
+
+NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel,
+                            const int64_t* kextent, int32_t border, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    Scratch tb;
+    NdArgs a;
+    int rc = prepare_nd(src, dst, kernel, kextent, border, tb, a, s);
+    if (rc || a.total == 0) return rc;
+    hipError_t e = launch_convolve_nd(a, src->dtype, dst->dtype, s);
     if (e != hipSuccess) return hip_fail(e, "convolve_nd launch");
+    return NDL_OK;
+}
+
+// ---------------------------------------------------------------- morphology
+NDL_API int ndl_morph_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* mask, const int64_t* kextent,
+                         int32_t border, int32_t op, void* stream)
+{
+    if (op != NDL_ERODE && op != NDL_DILATE) return fail(NDL_ERR_ARG, "unknown morphology op %d", op);
+    hipStream_t s = (hipStream_t)stream;
+    Scratch tb;
+    NdArgs a;
+    int rc = prepare_nd(src, dst, mask, kextent, border, tb, a, s);
+    if (rc || a.total == 0) return rc;
+    hipError_t e = launch_morph_nd(a, op == NDL_DILATE, src->dtype, dst->dtype, s);
+    if (e != hipSuccess) return hip_fail(e, "morph_nd launch");
+    return NDL_OK;
+}
+
+namespace {
+// one min/max pass along `axis`; corner (last pass of a float image): the
+// original source and, per tensor dimension, first-tap offset and border
+int run_morph_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& k, int dilate,
+                   const ndl_tensor* corner, const int32_t* coff_dim, const int32_t* cbord_dim, hipStream_t s)
+{
+    int64_t n = numel(in);
+    if (n == 0) return NDL_OK;
+    MorphPassArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = in->data;
+    a.out = out->data;
+    a.m = in->extent[axis];
+    a.in_as = in->stride[axis];
+    int order[NDL_MAX_DIM];
+    for (int d = 0; d < in->ndim; d++) order[d] = d;
+    std::stable_sort(order, order + in->ndim,
+                     [&](int x, int y) { return std::llabs(in->stride[x]) < std::llabs(in->stride[y]); });
+    a.nd = in->ndim;
+    for (int i = 0; i < in->ndim; i++) {
+        int d = order[i];
+        a.ext[i] = in->extent[d];
+        a.ist[i] = in->stride[d];
+        a.ost[i] = out->stride[d];
+        a.fdiv[i].init((uint32_t)in->extent[d]);
+        if (d == axis) a.axis_pos = i;
+        if (corner) {
+            a.cst[i] = corner->stride[d];
+            a.coff[i] = coff_dim[d];
+            a.cbord[i] = cbord_dim[d];
+        }
+    }
+    a.corner = corner ? corner->data : nullptr;
+    a.total = (uint32_t)n;
+    a.K = k.count;
+    a.center = k.center;
+    a.border = k.border;
+    a.dilate = dilate;
+    for (int t = 0; t < k.count; t++) a.mask[t] = k.taps[t] != 0.0;
+    hipError_t e = launch_morph_pass(a, in->dtype, out->dtype, s);
+    if (e != hipSuccess) return hip_fail(e, "morphology pass launch");
+    return NDL_OK;
+}
+} // namespace
+
+NDL_API int ndl_morph_separable(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+                                const ndl_axis_kernel* k, int32_t op, void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
+    if ((rc = check_same_shape(src, dst, "src", "dst"))) return rc;
+    if ((rc = check_axes(src, naxes, axes, k))) return rc;
+    if (op != NDL_ERODE && op != NDL_DILATE) return fail(NDL_ERR_ARG, "unknown morphology op %d", op);
+    if ((rc = check_no_overlap(src, dst))) return rc;
+    for (int i = 0; i < naxes; i++)
+        for (int j = 0; j < i; j++)
+            if (axes[i] == axes[j]) return fail(NDL_ERR_ARG, "axis %d listed twice", axes[i]);
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = numel(src);
+    if (n == 0) return NDL_OK;
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    const int nd = src->ndim;
+    // passes in ascending dimension order (ties resolve like the reference's
+    // kernel-coordinate order, dimension 0 fastest); per dimension the first
+    // included tap (the structuring element's corner) and its border
+    std::vector<int> ord(naxes);
+    for (int i = 0; i < naxes; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return axes[x] < axes[y]; });
+    int32_t coff[NDL_MAX_DIM] = {0}, cbord[NDL_MAX_DIM] = {0};
+    bool empty = false;
+    for (int i = 0; i < naxes; i++) {
+        int first = -1;
+        for (int t = 0; t < k[i].count && first < 0; t++)
+            if (k[i].taps[t] != 0.0) first = t;
+        if (first < 0) empty = true;
+        coff[axes[i]] = first - k[i].center;
+        cbord[axes[i]] = k[i].border;
+    }
+    if (empty) {
+        // no included tap: the reference writes SrcT{} everywhere (the
+        // one-pass kernel with an empty mask)
+        int64_t kext[NDL_MAX_DIM];
+        for (int d = 0; d < nd; d++) kext[d] = 1;
+        const double zero = 0.0;
+        return ndl_morph_nd(src, dst, &zero, kext, NDL_CLAMP, op, stream);
+    }
+    const bool fp = is_float_dt(src->dtype);
+    Scratch buf[2];
+    ndl_tensor mid[2];
+    for (int i = 0; i < 2 && i < naxes - 1; i++) {
+        if ((rc = alloc_scratch(buf[i], (size_t)n * dtype_size(src->dtype), s))) return rc;
+        mid[i] = dense_like(src, buf[i].p, src->dtype);
+    }
+    const ndl_tensor* cur = src;
+    for (int p = 0; p < naxes; p++) {
+        const int i = ord[p];
+        const bool last = p == naxes - 1;
+        const ndl_tensor* out = last ? dst : &mid[p & 1];
+        if ((rc = run_morph_pass(cur, out, axes[i], k[i], op == NDL_DILATE, last && fp ? src : nullptr, coff, cbord,
+                                 s)))
+            return rc;
+        cur = out;
+    }
     return NDL_OK;
 }
 
@@ -945,9 +1086,18 @@ struct PlanGroup {
     void* d_jobs = nullptr;
     void* d_ts = nullptr;
 };
+// one item as its own fused launch (compile-time-tap instances included)
+struct PlanSingle {
+    FusedArgs fa;
+    FusedConfig cfg;
+    int dt;
+    bool int_acc;
+};
 struct PlanVariant {
     int tz = 0;                 // z-chunk candidate
     std::vector<PlanGroup> groups;
+    bool singles = false;       // one fused launch per item instead of work queues
+    std::vector<PlanSingle> one;
     float ms = -1.f;            // measured time of one execute
     void release()
     {
@@ -986,6 +1136,10 @@ struct ndl_plan {
 namespace {
 int launch_groups(ndl_plan* p, const PlanVariant& v, hipStream_t s)
 {
+    for (const PlanSingle& o : v.one) {
+        hipError_t e = launch_fused(o.fa, o.cfg, o.dt, o.int_acc, s);
+        if (e != hipSuccess) return hip_fail(e, "fused launch");
+    }
     for (size_t i = 0; i < v.groups.size(); i++) {
         const PlanGroup& g = v.groups[i];
         FusedQueue q;
@@ -1046,6 +1200,22 @@ int build_variant(ndl_plan* p, int tz, PlanVariant& v, bool collect_fallback)
         if ((e = hipMemcpy(g.d_jobs, g.jobs.data(), jb, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(g.d_ts, g.tile_start.data(), tb, hipMemcpyHostToDevice)) != hipSuccess)
             return hip_fail(e, "plan table upload");
+    }
+    return NDL_OK;
+}
+
+// the per-item candidate: every fusable item as its own launch
+int build_singles(ndl_plan* p, PlanVariant& v)
+{
+    v.singles = true;
+    for (int i = 0; i < (int)p->items.size(); i++) {
+        ndl_batch_item& it = p->items[i];
+        if (numel(&it.src) == 0) continue;
+        PlanSingle o;
+        if (plan_fused(&it.src, &it.dst, it.naxes, it.axes, it.k, p->accum, o.fa, o.cfg, o.int_acc)) {
+            o.dt = it.src.dtype;
+            v.one.push_back(o);
+        }
     }
     return NDL_OK;
 }
@@ -1116,6 +1286,11 @@ NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accu
         if (rc) return rc;
         max_groups = std::max(max_groups, p->variants.back().groups.size());
     }
+    if (forced <= 0) {
+        p->variants.emplace_back();
+        int rc = build_singles(p.get(), p->variants.back());
+        if (rc) return rc;
+    }
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     p->grid = (uint32_t)std::max(1, cus) * (uint32_t)env_int("NDL_PLAN_WG_PER_CU", 4);
@@ -1134,7 +1309,7 @@ NDL_API int ndl_plan_execute(ndl_plan* p, void* stream)
         if (rc) return rc;
     }
     const PlanVariant& v = p->variants[p->chosen];
-    if (!v.groups.empty()) {
+    if (!v.groups.empty() || !v.one.empty()) {
         if (!p->tried_graph && env_int("NDL_PLAN_GRAPH", 1)) {
             // capture the work-queue launches once on a private stream
             p->tried_graph = true;

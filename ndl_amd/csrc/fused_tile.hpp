@@ -81,8 +81,11 @@ __device__ __forceinline__ void fma_block(float (&acc)[NO], const float (&win)[N
 }
 
 // 1-D correlation of NO consecutive outputs read through get(i) (window
-// element i).  KS > 0: KS taps, fully unrolled.  KS == 0: K taps at run time,
-// 4-tap chunks then a 1..3-tap tail.
+// element i).  KS > 0: KS taps, fully unrolled.  KS == 0: K taps at run time
+// in 4-tap chunks over a sliding window: chunk [t0, t0+4) needs elements
+// [t0, t0 + NO + 3), of which the first NO - 1 are carried over from the
+// previous chunk, so every element is read once and nothing past the window
+// (NO + K - 1 elements) is touched.
 template <int NO, int KS, int IL = 1, class Get>
 __device__ __forceinline__ void correlate(float (&acc)[NO], int K, const float* w, Get get)
 {
@@ -95,18 +98,22 @@ __device__ __forceinline__ void correlate(float (&acc)[NO], int K, const float* 
         fma_block<NO, KS, NW, IL>(acc, win, w);
     } else {
         constexpr int NW = NO + 3;
+        float win[NW];
+#pragma unroll
+        for (int i = 0; i < NO - 1; i++) win[i] = get(i);
         int t0 = 0;
         for (; t0 + 4 <= K; t0 += 4) {
-            float win[NW];
 #pragma unroll
-            for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
+            for (int i = NO - 1; i < NW; i++) win[i] = get(t0 + i);
             fma_block<NO, 4, NW>(acc, win, w + t0);
+#pragma unroll
+            for (int i = 0; i < NO - 1; i++) win[i] = win[i + 4];
         }
         const int rem = K - t0;
         if (rem > 0) {
-            float win[NW];
 #pragma unroll
-            for (int i = 0; i < NW; i++) win[i] = get(t0 + i);
+            for (int i = NO - 1; i < NO + 2; i++)
+                if (i < NO - 1 + rem) win[i] = get(t0 + i);
             if (rem == 3) fma_block<NO, 3, NW>(acc, win, w + t0);
             else if (rem == 2) fma_block<NO, 2, NW>(acc, win, w + t0);
             else fma_block<NO, 1, NW>(acc, win, w + t0);
@@ -319,34 +326,35 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
         for (int p = 0; p < P; p++) ring[o][p] = 0.f;
 
     // y-pass of one slice: window `in` -> y-filtered plane `yp`
-    auto y_pass = [&](const float* in, float* yp) NDL_INLINE {
+    // one y-pass task: column c of the window, output rows [r0, r0 + RY)
+    auto y_task = [&](const float* in, float* yp, int r0, int c) NDL_INLINE {
+        const float* col = in + r0 * NC + c;
+        float o[RY];
 #pragma unroll
-        for (int k = 0; k < NTASKMAX; k++) {
-            if (tid + k * NT < ntask) {
-                const float* col = in + trow[k] * NC + tcol[k];
-                float o[RY];
+        for (int r = 0; r < RY; r++) o[r] = 0.f;
+        correlate<RY, KYS>(o, ky, wyp, [&](int i) NDL_INLINE { return col[i * NC]; });
+        if constexpr (EDGE) {
+            if (ey) {
 #pragma unroll
-                for (int r = 0; r < RY; r++) o[r] = 0.f;
-                correlate<RY, KYS>(o, ky, wyp, [&](int i) NDL_INLINE { return col[i * NC]; });
-                if constexpr (EDGE) {
-                    if (ey) {
-#pragma unroll
-                        for (int r = 0; r < RY; r++) {
-                            const int yy = y0 + trow[k] + r;
-                            const int ri = yy < a.Y ? edge_row(yy, a.Y, a.eL[1], a.eR[1]) : -1;
-                            if (ri >= 0) {
-                                const float* w = sE + eoff_y + ri * ky;
-                                float acc = 0.f;
-                                for (int t = 0; t < ky; t++) acc = __builtin_fmaf(col[(r + t) * NC], w[t], acc);
-                                o[r] = acc;
-                            }
-                        }
+                for (int r = 0; r < RY; r++) {
+                    const int yy = y0 + r0 + r;
+                    const int ri = yy < a.Y ? edge_row(yy, a.Y, a.eL[1], a.eR[1]) : -1;
+                    if (ri >= 0) {
+                        const float* w = sE + eoff_y + ri * ky;
+                        float acc = 0.f;
+                        for (int t = 0; t < ky; t++) acc = __builtin_fmaf(col[(r + t) * NC], w[t], acc);
+                        o[r] = acc;
                     }
                 }
-#pragma unroll
-                for (int r = 0; r < RY; r++) yp[(trow[k] + r) * NCP + tcol[k]] = o[r];
             }
         }
+#pragma unroll
+        for (int r = 0; r < RY; r++) yp[(r0 + r) * NCP + c] = o[r];
+    };
+    auto y_pass = [&](const float* in, float* yp) NDL_INLINE {
+#pragma unroll
+        for (int k = 0; k < NTASKMAX; k++)
+            if (tid + k * NT < ntask) y_task(in, yp, trow[k], tcol[k]);
     };
     // x-pass of one y-filtered plane, then ring slot U (slice s)
     auto xz_pass = [&](auto slot, int s, const float* yp) NDL_INLINE {

@@ -1,5 +1,7 @@
 // kernels_fused.hip -- forward instances of the fused separable kernel
 // (fused_tile.hpp) and the launch / occupancy dispatch.
+#include <cstdlib>
+
 #include "fused_tile.hpp"
 
 namespace ndl {
@@ -27,6 +29,9 @@ template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int 
         return hipErrorInvalidValue;
     }
     if (cfg.ty != kTY) return hipErrorInvalidValue;
+    // NDL_FUSED_GENERIC=1: run-time-tap instances only (measurement knob)
+    static const bool generic_only = getenv("NDL_FUSED_GENERIC") && atoi(getenv("NDL_FUSED_GENERIC")) == 1;
+    if (generic_only) goto generic;
 #define NDL_SPEC_IL(DT, T, C, KX, KY, KZ, IL)                                                 \
     if (dt == DT && cpt == C && kx == KX && ky == KY && kz == KZ && il == IL)                 \
         return f(T{}, IC<C>{}, IC<KX>{}, IC<KY>{}, IC<KZ>{}, IC<IL>{}, IC<kTY>{});
@@ -44,6 +49,7 @@ template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int 
     NDL_SPEC_IL(NDL_U8, uint8_t, 2, 7, 7, 1, 3)
 #undef NDL_SPEC
 #undef NDL_SPEC_IL
+generic:
     if (cpt != 2 || il != 1) return hipErrorInvalidValue;
 #define NDL_GEN(DT, T, KZ) \
     if (dt == DT && kz == KZ) return f(T{}, IC<2>{}, IC<0>{}, IC<0>{}, IC<KZ>{}, IC<1>{}, IC<kTY>{});

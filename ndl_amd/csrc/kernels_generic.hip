@@ -156,12 +156,146 @@ hipError_t launch_convolve_nd(const NdArgs& a, int in_dt, int out_dt, hipStream_
     });
 }
 
+// ---------------------------------------------------------- morphology
+// Comparison key of a storage type: integers as themselves, half formats as
+// float (exact and order-preserving); the winner is kept in its storage type
+// so a same-type output copies it bit for bit (NaN payloads, signed zeros).
+template <class T> struct MorphKey { using type = T; };
+template <> struct MorphKey<f16_t> { using type = float; };
+template <> struct MorphKey<bf16_t> { using type = float; };
+template <class T> __device__ __forceinline__ typename MorphKey<T>::type morph_key(T v) { return v; }
+template <> __device__ __forceinline__ float morph_key<f16_t>(f16_t v) { return f16_to_f32(v.bits); }
+template <> __device__ __forceinline__ float morph_key<bf16_t>(bf16_t v) { return bf16_to_f32(v.bits); }
+template <class K> __device__ __forceinline__ bool key_nan(K v)
+{
+    if constexpr (std::is_floating_point<K>::value) return v != v;
+    else return false;
+}
+
+template <class TIn, class TOut> __device__ __forceinline__ void morph_store(TOut* p, TIn v)
+{
+    if constexpr (std::is_same<TIn, TOut>::value) *p = v;
+    else store_from_f64(p, load_f64(&v), NDL_ROUND_TRUNC); // static_cast<DstT>(best)
+}
+
+template <class TIn, class TOut, bool MAX>
+__global__ __launch_bounds__(256) void morph_pass_kernel(const MorphPassArgs a)
+{
+    using Key = typename MorphKey<TIn>::type;
+    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.total) return;
+    int64_t ioff = 0, ooff = 0, coff = 0, j = 0;
+    uint32_t rem = idx;
+    for (int d = 0; d < a.nd; d++) {
+        uint32_t q = a.fdiv[d].div(rem);
+        int64_t c = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+        rem = q;
+        ioff += c * a.ist[d];
+        ooff += c * a.ost[d];
+        if (d == a.axis_pos) j = c;
+        if (a.corner) {
+            int64_t x = c + a.coff[d];
+            if (x < 0 || x >= a.ext[d]) x = border_map(a.cbord[d], a.ext[d], x);
+            coff += x * a.cst[d];
+        }
+    }
+    const TIn* in = (const TIn*)a.in + (ioff - j * a.in_as);
+    bool have = false;
+    TIn best{};
+    Key bk{};
+    for (int t = 0; t < a.K; t++) {
+        if (!a.mask[t]) continue; // uniform
+        int64_t x = j + t - a.center;
+        if (x < 0 || x >= a.m) x = border_map(a.border, a.m, x);
+        const TIn v = in[x * a.in_as];
+        const Key k = morph_key(v);
+        // strict comparison, first tap wins ties; NaN never wins over a number
+        const bool take = !have || (key_nan(bk) ? !key_nan(k) : (MAX ? k > bk : k < bk));
+        if (take) {
+            best = v;
+            bk = k;
+            have = true;
+        }
+    }
+    if constexpr (std::is_floating_point<Key>::value) {
+        if (a.corner) {
+            const TIn cv = ((const TIn*)a.corner)[coff];
+            if (key_nan(morph_key(cv))) best = cv;
+        }
+    }
+    morph_store<TIn, TOut>((TOut*)a.out + ooff, best);
+}
+
+hipError_t launch_morph_pass(const MorphPassArgs& a, int in_dt, int out_dt, hipStream_t s)
+{
+    if (a.total == 0) return hipSuccess;
+    uint32_t blocks = (a.total + 255u) / 256u;
+    return with_dtype(in_dt, [&](auto tin) {
+        using TIn = decltype(tin);
+        return with_dtype(out_dt, [&](auto tout) {
+            using TOut = decltype(tout);
+            if (a.dilate) hipLaunchKernelGGL((morph_pass_kernel<TIn, TOut, true>), dim3(blocks), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((morph_pass_kernel<TIn, TOut, false>), dim3(blocks), dim3(256), 0, s, a);
+            return hipGetLastError();
+        });
+    });
+}
+
+// the reference's own loop: first included tap, then strictly smaller
+// (erode) / larger (dilate) values replace it; SrcT{} without taps
+template <class TIn, class TOut, bool MAX>
+__global__ __launch_bounds__(256) void morph_nd_kernel(const NdArgs a)
+{
+    using Key = typename MorphKey<TIn>::type;
+    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.total) return;
+    int64_t coord[NDL_MAX_DIM];
+    int64_t ooff = 0;
+    uint32_t rem = idx;
+    for (int d = 0; d < a.nd; d++) {
+        uint32_t q = a.fdiv[d].div(rem);
+        coord[d] = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+        rem = q;
+        ooff += coord[d] * a.ost[d];
+    }
+    TIn best{};
+    Key bk{};
+    for (int t = 0; t < a.ntaps; t++) {
+        int64_t off = 0;
+        for (int d = 0; d < a.nd; d++) {
+            int64_t x = coord[d] + a.tap_off[t * a.nd + d];
+            if (x < 0 || x >= a.ext[d]) x = border_map(a.border, a.ext[d], x);
+            off += x * a.ist[d];
+        }
+        const TIn v = ((const TIn*)a.in)[off];
+        const Key k = morph_key(v);
+        if (t == 0 || (MAX ? k > bk : k < bk)) {
+            best = v;
+            bk = k;
+        }
+    }
+    morph_store<TIn, TOut>((TOut*)a.out + ooff, best);
+}
+
+hipError_t launch_morph_nd(const NdArgs& a, int dilate, int in_dt, int out_dt, hipStream_t s)
+{
+    if (a.total == 0) return hipSuccess;
+    uint32_t blocks = (a.total + 255u) / 256u;
+    return with_dtype(in_dt, [&](auto tin) {
+        using TIn = decltype(tin);
+        return with_dtype(out_dt, [&](auto tout) {
+            using TOut = decltype(tout);
+            if (dilate) hipLaunchKernelGGL((morph_nd_kernel<TIn, TOut, true>), dim3(blocks), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((morph_nd_kernel<TIn, TOut, false>), dim3(blocks), dim3(256), 0, s, a);
+            return hipGetLastError();
+        });
+    });
+}
+
 // ------------------------------------------------------ tap gradients
-// blockIdx.y = tap.  Each block walks a fixed, strided share of positions,
-// reduces in FP64 through LDS in a fixed tree, so the sum is deterministic.
 // Tap gradient of one axis: grad[t] = sum_p g[p] * U[p with its coordinate
 // j along the axis replaced by B(j + t - c)].  Work item = (segment of
-// kTapSeg consecutive positions along the axis, line); lanes take
+// kTapSeg consecutive positions along the axis, line); with SEG, lanes take
 // consecutive lines (coalesced along the fastest other dimension) and share
 // the segment, so the border test is uniform.  A thread loads the U window
 // of its segment once (kTapSeg + K - 1 values) and its kTapSeg g values, and
@@ -170,8 +304,10 @@ hipError_t launch_convolve_nd(const NdArgs& a, int in_dt, int out_dt, hipStream_
 // reduced by wave shuffles then across the four waves in LDS;
 // tapgrad_final_kernel sums the partials in a fixed tree order, so the result
 // does not depend on scheduling.
-
-template <class TG, class TU, int KB>
+// SEG = false (the axis is the fastest dimension of g): lanes take
+// consecutive segments of one line instead, so each lane reads a contiguous
+// run and the wave a contiguous block.
+template <class TG, class TU, int KB, bool SEG>
 __global__ __launch_bounds__(256) void tapgrad_partial_kernel(const TapGradArgs a)
 {
     constexpr int NW = kTapSeg + KB - 1;
@@ -180,8 +316,14 @@ __global__ __launch_bounds__(256) void tapgrad_partial_kernel(const TapGradArgs 
     for (int t = 0; t < KB; t++) acc[t] = 0.0;
     const uint32_t step = 256u * gridDim.x;
     for (uint32_t item = blockIdx.x * 256u + threadIdx.x; item < a.items; item += step) {
-        const uint32_t seg = a.ldiv.div(item);
-        uint32_t rem = item - seg * a.lines;
+        uint32_t seg, rem;
+        if constexpr (SEG) {
+            seg = a.ldiv.div(item);
+            rem = item - seg * a.lines;
+        } else {
+            rem = a.ldiv.div(item);
+            seg = item - rem * a.nseg;
+        }
         int64_t goff = 0, uoff = 0;
         for (int d = 0; d < a.nl; d++) {
             uint32_t q = a.fdiv[d].div(rem);
@@ -262,7 +404,10 @@ hipError_t launch_tapgrad_partial(const TapGradArgs& a, int g_dt, int u_is_f64, 
             using TG = decltype(tg);
 #define NDL_TG(KB)                                                                                          \
     if (a.K <= KB) {                                                                                        \
-        hipLaunchKernelGGL((tapgrad_partial_kernel<TG, TU, KB>), dim3(a.nblocks), dim3(256), 0, s, a);       \
+        if (a.seg)                                                                                          \
+            hipLaunchKernelGGL((tapgrad_partial_kernel<TG, TU, KB, true>), dim3(a.nblocks), dim3(256), 0, s, a); \
+        else                                                                                                \
+            hipLaunchKernelGGL((tapgrad_partial_kernel<TG, TU, KB, false>), dim3(a.nblocks), dim3(256), 0, s, a); \
         return hipGetLastError();                                                                           \
     }
             NDL_TG(4) NDL_TG(8) NDL_TG(16) NDL_TG(32) NDL_TG(64)

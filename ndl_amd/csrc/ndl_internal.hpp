@@ -53,10 +53,42 @@ struct NdArgs {
 };
 hipError_t launch_convolve_nd(const NdArgs& a, int in_dt, int out_dt, hipStream_t s);
 
+// --------------------------------------------------------- morphology
+// One min (erode) / max (dilate) pass along one axis over the taps whose
+// mask entry is nonzero (processing/morphology.h:57-101).  Floating-point
+// passes skip NaN (a NaN is kept only when every value is NaN); the last
+// pass then applies the reference's rule for the first included tap of the
+// whole structuring element (the window corner): when it is NaN the result
+// is that NaN.  With strict comparisons in ascending tap order and passes in
+// ascending dimension order, ties resolve to the first tap in kernel
+// coordinate order, as the reference's single loop does.
+struct MorphPassArgs {
+    const void* in;
+    void* out;
+    int64_t m, in_as;
+    int32_t nd, axis_pos;
+    int64_t ext[NDL_MAX_DIM];
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t total;
+    int32_t K, center, border;
+    int32_t dilate;               // 0: minimum, 1: maximum
+    uint8_t mask[NDL_MAX_TAPS];   // included taps of this axis
+    const void* corner;           // original source for the NaN rule, or null
+    int64_t cst[NDL_MAX_DIM];     // its strides (iteration order)
+    int32_t coff[NDL_MAX_DIM];    // first included tap - centre per dimension (0: none)
+    int32_t cbord[NDL_MAX_DIM];   // border per dimension
+};
+hipError_t launch_morph_pass(const MorphPassArgs& a, int in_dt, int out_dt, hipStream_t s);
+// erode/dilate with an N-D mask in one pass, the reference's loop exactly
+// (NdArgs::tap_off lists the included taps; tap_val is unused)
+hipError_t launch_morph_nd(const NdArgs& a, int dilate, int in_dt, int out_dt, hipStream_t s);
+
 // ------------------------------------------- tap-gradient partial sums
 // partial[blk*K + t] = sum over the block's share of positions p of
 // g[p] * u[B(p_axis + t - c)] (FP64), later summed in block order.
-constexpr int kTapSeg = 8;     // tap-gradient positions per work item along the axis
+constexpr int kTapSeg = 16;    // tap-gradient positions per work item along the axis
 struct TapGradArgs {
     const void* g;      // grad_out, strided
     const void* u;      // dense intermediate (Acc type)
@@ -67,8 +99,10 @@ struct TapGradArgs {
     int64_t ust[NDL_MAX_DIM];
     FastDiv fdiv[NDL_MAX_DIM];
     uint32_t lines;                // product of the other extents
-    uint32_t items;                // lines * segments
-    FastDiv ldiv;                  // item -> (segment, line)
+    int32_t seg;                   // 1: item = segment * lines + line; 0: line * nseg + segment
+    uint32_t nseg;                 // segments per line
+    uint32_t items;                // lines * nseg
+    FastDiv ldiv;                  // divides by lines (seg = 1) or by nseg (seg = 0)
     int64_t m, g_as, u_as;         // axis extent and strides
     int32_t K, center, border;
     double* partial;    // [nblocks][K]

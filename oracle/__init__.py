@@ -40,6 +40,7 @@ def _load():
         raise RuntimeError(f"{LIB_PATH} is not built (run make -C oracle)")
     lib = C.CDLL(LIB_PATH)
     abi.bind(lib, "oracle_convolve", C.c_int, _T, _T, _DP, _I64P, C.c_int)
+    abi.bind(lib, "oracle_morph", C.c_int, _T, _T, _DP, _I64P, C.c_int, C.c_int)
     abi.bind(lib, "oracle_sepconv", C.c_int, _T, _T, C.c_int, _I32P, _K, C.c_int)
     abi.bind(lib, "oracle_sepconv_adjoint_input", C.c_int, _T, _T, C.c_int, _I32P, _K)
     abi.bind(lib, "oracle_sepconv_adjoint_taps", C.c_int, _T, _T, C.c_int, _I32P, _K, _DP)
@@ -61,6 +62,8 @@ def _load_ref():
     abi.bind(lib, "ref_gaussian_blur", C.c_int, _T, _T, C.c_double, C.c_int)
     abi.bind(lib, "ref_sepconv", C.c_int, _T, _T, C.c_int, _I32P, _K, C.c_int)
     abi.bind(lib, "ref_kernel_size", C.c_int, C.c_double)
+    if hasattr(lib, "ref_morph"):
+        abi.bind(lib, "ref_morph", C.c_int, _T, _T, _DP, _I64P, C.c_int, C.c_int)
     return lib
 
 
@@ -111,6 +114,25 @@ def convolve(src, kernel, border="clamp", dst=None, src_bf16=False, dst_bf16=Fal
         _check(ref_lib.ref_convolve(C.byref(ts), C.byref(td), kp, kext, _border(border)), "ref_convolve")
     else:
         _check(lib.oracle_convolve(C.byref(ts), C.byref(td), kp, kext, _border(border)), "oracle_convolve")
+    return dst
+
+
+def morph(src, kernel, op, border="clamp", dst=None, use_ref=False):
+    """ndl::erode (op "erode") / ndl::dilate (op "dilate") with an N-D
+    structuring element (numpy order; nonzero = included),
+    morphology.h:57-101."""
+    kernel = np.ascontiguousarray(kernel, dtype=np.float64)
+    assert kernel.ndim == src.ndim and op in ("erode", "dilate")
+    if dst is None:
+        dst = _out_like(src, None)
+    kext = (C.c_int64 * abi.NDL_MAX_DIM)(*reversed(kernel.shape))
+    ts, td = abi.host_tensor(src), abi.host_tensor(dst)
+    kp = kernel.ctypes.data_as(_DP)
+    o = 0 if op == "erode" else 1
+    if use_ref:
+        _check(ref_lib.ref_morph(C.byref(ts), C.byref(td), kp, kext, _border(border), o), "ref_morph")
+    else:
+        _check(lib.oracle_morph(C.byref(ts), C.byref(td), kp, kext, _border(border), o), "oracle_morph")
     return dst
 
 

@@ -24,6 +24,11 @@
  *                           reference's compiled code does)
  *   oracle_sepconv_adjoint_input / _taps: no reference counterpart; the exact
  *                           transpose / tap derivative of oracle_sepconv in FP64.
+ *   oracle_morph            processing/morphology.h:57-101 (erode / dilate:
+ *                           first included tap in kernel-coordinate order,
+ *                           then strictly smaller / larger values replace it;
+ *                           static_cast<DstT> of the winner), same tap list
+ *                           and border resolution as oracle_convolve.
  *
  * How each is pinned (tests/test_oracle.py): the reference unit-test and
  * tutorial golden values (image_convolution_tests.cpp:13-67,
@@ -462,6 +467,55 @@ EXPORT int oracle_sepconv_adjoint_taps(const ndl_tensor* x, const ndl_tensor* g,
 }
 
 /* element codecs, exposed so tests can check the storage conversions */
+/* erode (op 0) / dilate (op 1), morphology.h:57-101.  Values compare as
+ * doubles: every storage type converts exactly and in order, NaN and signed
+ * zeros included, so `v < best` is the reference's comparison. */
+EXPORT int oracle_morph(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel,
+                        const int64_t* kextent, int border, int op)
+{
+    int nd = src->ndim;
+    if (nd < 1 || nd > NDL_MAX_DIM || dst->ndim != nd) return NDL_ERR_SHAPE;
+    if (op != 0 && op != 1) return NDL_ERR_ARG;
+    int64_t center[NDL_MAX_DIM], kn = 1;
+    for (int d = 0; d < nd; d++) {
+        if (src->extent[d] != dst->extent[d] || kextent[d] < 1) return NDL_ERR_SHAPE;
+        center[d] = kextent[d] / 2;
+        if (!reach_ok(border, src->extent[d], center[d], kextent[d] - 1 - center[d])) return NDL_ERR_RADIUS;
+        kn *= kextent[d];
+    }
+    int64_t* tapc = (int64_t*)malloc(sizeof(int64_t) * (size_t)(kn * nd + 1));
+    int64_t ntaps = 0;
+    for (int64_t lin = 0; lin < kn; lin++) {
+        if (kernel[lin] == 0.0) continue;
+        int64_t r = lin;
+        for (int d = 0; d < nd; d++) {
+            tapc[ntaps * nd + d] = r % kextent[d] - center[d];
+            r /= kextent[d];
+        }
+        ntaps++;
+    }
+    int64_t n = numel(src), coord[NDL_MAX_DIM] = {0};
+    for (int64_t lin = 0; lin < n; lin++) {
+        double best = 0; /* SrcT{} when no tap is included */
+        for (int64_t t = 0; t < ntaps; t++) {
+            int64_t off = 0;
+            for (int d = 0; d < nd; d++)
+                off += border_map(border, src->extent[d], coord[d] + tapc[t * nd + d]) * src->stride[d];
+            double v = load_elem(src->data, src->dtype, off);
+            if (t == 0 || (op == 0 ? v < best : v > best)) best = v;
+        }
+        int64_t doff = 0;
+        for (int d = 0; d < nd; d++) doff += coord[d] * dst->stride[d];
+        store_elem(dst->data, dst->dtype, doff, best, NDL_ROUND_TRUNC);
+        for (int d = 0; d < nd; d++) {
+            if (++coord[d] < src->extent[d]) break;
+            coord[d] = 0;
+        }
+    }
+    free(tapc);
+    return NDL_OK;
+}
+
 EXPORT double oracle_half_to_double(uint16_t h) { return half_to_double(h); }
 EXPORT uint16_t oracle_double_to_half(double v) { return double_to_half(v); }
 EXPORT double oracle_bf16_to_double(uint16_t b) { return bf16_to_double(b); }

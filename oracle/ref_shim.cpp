@@ -8,7 +8,8 @@
 // this file only instantiates ndl::convolve / ndl::gaussian_blur
 // (processing/convolution.h:52-71,107-186) on dense Image<T,DIM> views
 // (image/core.h:228-235) and composes per-axis passes the way a caller of the
-// reference would (double intermediates, as convolution.h:151-185 does).
+// reference would (double intermediates, as convolution.h:151-185 does);
+// ndl::erode / ndl::dilate (processing/morphology.h:57-101) likewise.
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "processing/convolution.h"
+#include "processing/morphology.h"
 
 #include "../include/ndl_gpu.h"
 
@@ -140,6 +142,33 @@ int dispatch_dim_sep(const ndl_tensor* s, const ndl_tensor* d, int na, const int
     return NDL_ERR_SHAPE;
 }
 
+template <class T, int DIM>
+int morph_t(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel, const int64_t* kextent,
+            int border, int op)
+{
+    ndl::Image<T, DIM> s((T*)src->data, ext_of<DIM>(src));
+    ndl::Image<T, DIM> d((T*)dst->data, ext_of<DIM>(dst));
+    std::array<int, DIM> ke{};
+    for (int i = 0; i < DIM; i++) ke[i] = (int)kextent[i];
+    ndl::Image<double, DIM> k(const_cast<double*>(kernel), ke);
+    if (op == 0) ndl::erode(s, d, k, bmode(border));
+    else ndl::dilate(s, d, k, bmode(border));
+    return 0;
+}
+
+template <class T>
+int dispatch_dim_morph(const ndl_tensor* s, const ndl_tensor* d, const double* k, const int64_t* ke, int b,
+                       int op)
+{
+    switch (s->ndim) {
+    case 1: return morph_t<T, 1>(s, d, k, ke, b, op);
+    case 2: return morph_t<T, 2>(s, d, k, ke, b, op);
+    case 3: return morph_t<T, 3>(s, d, k, ke, b, op);
+    case 4: return morph_t<T, 4>(s, d, k, ke, b, op);
+    }
+    return NDL_ERR_SHAPE;
+}
+
 bool dense_same(const ndl_tensor* s, const ndl_tensor* d)
 {
     if (s->dtype != d->dtype || s->ndim != d->ndim) return false;
@@ -192,6 +221,15 @@ __attribute__((visibility("default"))) int ref_sepconv(const ndl_tensor* src, co
 {
     if (!dense_same(src, dst) || naxes < 1) return NDL_ERR_ARG;
     NDL_REF_DISPATCH(dispatch_dim_sep, src, dst, naxes, axes, k, rounding)
+}
+
+// erode() (op 0) / dilate() (op 1) on dense same-type src/dst
+__attribute__((visibility("default"))) int ref_morph(const ndl_tensor* src, const ndl_tensor* dst,
+                                                     const double* kernel, const int64_t* kextent, int border,
+                                                     int op)
+{
+    if (!dense_same(src, dst) || (op != 0 && op != 1)) return NDL_ERR_ARG;
+    NDL_REF_DISPATCH(dispatch_dim_morph, src, dst, kernel, kextent, border, op)
 }
 
 // mathHelpers.h _kernelSize
