@@ -246,7 +246,7 @@ def _time_launches(hip, stream, fn, reps, flush=None):
     return float(np.mean([hip.elapsed_ms(a, b) for a, b in ts]))
 
 
-def suite(reps=10, configs=(1, 2, 3, 4, 5)):
+def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
     """Secondary configurations; one JSON line each on stdout."""
     import ndl_amd
     import oracle
@@ -368,12 +368,19 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5)):
 
                 def adj():
                     ndl_amd.adjoint_input_dev(dw, dgx, axes, taps, borders, abi.ACCUM_F32, stream=stream)
+
+                dgt = ndl_amd.DeviceArray((27,), np.float64)
+
+                def adj_taps():
+                    ndl_amd.adjoint_taps_dev(d32, dw, axes, taps, borders, dgt, abi.ACCUM_F32, stream=stream)
                 f_half()
                 f_32()
                 adj()
                 ms_h = _time_launches(hip, stream, f_half, reps, flush)
                 ms_32 = _time_launches(hip, stream, f_32, reps, flush)
                 ms_adj = _time_launches(hip, stream, adj, reps, flush)
+                adj_taps()
+                ms_taps = _time_launches(hip, stream, adj_taps, max(3, reps // 2), flush)
                 # errors vs the FP64 oracle fed the storage-rounded input
                 want = oracle.sepconv(xd, axes, taps, borders, dst=np.zeros(xd.shape))
                 got = dy.to_numpy()
@@ -385,15 +392,46 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5)):
                 emit({"config": f"5: {storage} 384^3, 3x9 random taps, {border}",
                       "fwd_us": round(ms_h * 1e3, 2), "fwd_f32_form_us": round(ms_32 * 1e3, 2),
                       "speedup_vs_f32": round(ms_32 / ms_h, 3), "adj_input_f32_us": round(ms_adj * 1e3, 2),
+                      "adj_taps_f32_us": round(ms_taps * 1e3, 1),
                       "fwd_max_abs_err": float(np.max(np.abs(err))), "fwd_rms_err": float(np.sqrt(np.mean(err ** 2))),
                       "fwd_rms_ref": float(np.sqrt(np.mean(want ** 2))),
                       "adj_max_abs_err": float(np.max(np.abs(egx))), "adj_rms_err": float(np.sqrt(np.mean(egx ** 2)))})
-                del dx, dy, dw, dgx, d32, y32
+                del dx, dy, dw, dgx, d32, y32, dgt
+    if 6 in configs:
+        # separable morphology (box structuring elements), bit-exact vs the oracle
+        cases = [("6: u8 2048x2048 box 7x7 erode, clamp", (2048, 2048), np.uint8, [7, 7], "clamp", "erode"),
+                 ("6: f32 256^3 box 5x5x5 dilate, reflect", (256, 256, 256), np.float32, [5, 5, 5], "reflect",
+                  "dilate")]
+        for name, shape, dt, ks_, border, op in cases:
+            n = int(np.prod(shape))
+            if dt == np.uint8:
+                x = workloads.uniform_ints(3, 9, n, 0, 255).astype(np.uint8).reshape(shape)
+            else:
+                x = workloads.uniform(2, 9, n).astype(np.float32).reshape(shape)
+            s_ = ndl_amd.DeviceArray.from_numpy(x)
+            d_ = ndl_amd.DeviceArray(shape, dt)
+            axes = list(range(len(shape)))
+            kk = abi.Kernels([np.ones(k) for k in ks_], [border] * len(ks_))
+            ax = abi.Kernels.axes(axes)
+            ts_, td_ = s_.tensor(), d_.tensor()
+
+            def f(ts_=ts_, td_=td_, kk=kk, ax=ax, nax=len(axes), op=op):
+                ndl_amd.check(lib.ndl_morph_separable(C.byref(ts_), C.byref(td_), nax, ax, kk.arr,
+                                                      ndl_amd.MORPH_OPS[op], stream))
+            f()
+            ms = _time_launches(hip, stream, f, reps, flush)
+            ok = True
+            if n <= 1 << 22:
+                ok = bool(np.array_equal(d_.to_numpy(), oracle.morph(x, np.ones(ks_), op, border)))
+            emit({"config": name, "us": round(ms * 1e3, 2), "gvox_s": round(n / ms / 1e6, 2),
+                  "GBps_alg": round(2 * x.itemsize * n / ms / 1e6, 1), "passes": len(ks_),
+                  "bit_exact": ok if n <= 1 << 22 else "checked in tests at smaller sizes"})
+            del s_, d_
 
 
 if __name__ == "__main__":
     if "--suite" in sys.argv:
-        which = [int(a) for a in sys.argv[sys.argv.index("--suite") + 1:] if a.isdigit()] or [1, 2, 3, 4, 5]
+        which = [int(a) for a in sys.argv[sys.argv.index("--suite") + 1:] if a.isdigit()] or [1, 2, 3, 4, 5, 6]
         suite(configs=tuple(which))
     else:
         main()

@@ -288,6 +288,23 @@ public:
 /// Fills `kernel` with ones: the box kernel (image/kernels.h:22-23).
 template <class T, int DIM> void make_box_kernel(Image<T, DIM>& kernel) { kernel = T(1); }
 
+/// Plus-sign structuring element (image/kernels.h:44-58): the centre and,
+/// per axis, the taps that differ from the centre along that axis only.
+template <class T, int DIM> void make_cross_kernel(Image<T, DIM>& kernel)
+{
+    kernel = T(0);
+    std::array<int, DIM> centre;
+    for (int i = 0; i < DIM; i++) centre[i] = kernel.extent()[i] / 2;
+    kernel.at(centre) = T(1);
+    for (int axis = 0; axis < DIM; axis++) {
+        std::array<int, DIM> c = centre;
+        for (int i = 0; i < kernel.extent()[axis]; i++) {
+            c[axis] = i;
+            kernel.at(c) = T(1);
+        }
+    }
+}
+
 /// Runs fn(srcChannel, dstChannel) on every slice along channelAxis
 /// (image/kernels.h:78-87).
 template <class T, int DIM, class Fn>

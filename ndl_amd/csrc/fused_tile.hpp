@@ -200,7 +200,13 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     const int NCP = (NC + 3 + 3) & ~3;
     const int NIN = NR * NC;
     // two input windows and two y-filtered planes (pipelined steps)
-    const int IN_FL = ((NR + 3) * NC + 3) & ~3;
+    // PAD (compile-time taps): every load slot and every lane of the y-pass
+    // waves works unconditionally -- surplus load lanes fetch element 0 into
+    // LDS past the window, surplus y lanes recompute a task of the previous
+    // wave (the same value to the same address) -- so the per-step work has
+    // no lane-divergent branches
+    constexpr bool PAD = KXS > 0 && KYS > 0;
+    const int IN_FL = PAD ? max(((NR + 3) * NC + 3) & ~3, NLMAX * NT) : ((NR + 3) * NC + 3) & ~3;
     const int Y_FL = TY * NCP;
     float* sIn0 = (float*)smem_raw;
     float* sIn1 = sIn0 + IN_FL;
@@ -269,7 +275,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     int tcol[NTASKMAX], trow[NTASKMAX];
 #pragma unroll
     for (int k = 0; k < NTASKMAX; k++) {
-        const int task = tid + k * NT;
+        int task = tid + k * NT;
+        if (PAD && NTASKMAX == 1 && task >= NC * (TY / RY) && NC * (TY / RY) >= 64) task -= 64;
         const int rg = task / NC;
         tcol[k] = task - rg * NC;
         trow[k] = rg * RY;
@@ -316,7 +323,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
         const TS* base = src + (int64_t)zz * a.sz;
 #pragma unroll
         for (int k = 0; k < NLMAX; k++)
-            if (tid + k * NT < NIN) dstv[k] = load_s<TS>(base + loff[k]);
+            if (PAD || tid + k * NT < NIN) dstv[k] = load_s<TS>(base + loff[k]);
     };
 
     float ring[CPT][P];
@@ -353,8 +360,14 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     };
     auto y_pass = [&](const float* in, float* yp) NDL_INLINE {
 #pragma unroll
-        for (int k = 0; k < NTASKMAX; k++)
-            if (tid + k * NT < ntask) y_task(in, yp, trow[k], tcol[k]);
+        for (int k = 0; k < NTASKMAX; k++) {
+            if constexpr (PAD && NTASKMAX == 1) {
+                // whole waves: the first ceil(ntask / 64) take a task each
+                if (__builtin_amdgcn_readfirstlane(tid >> 6) * 64 < ntask) y_task(in, yp, trow[k], tcol[k]);
+            } else if (tid + k * NT < ntask) {
+                y_task(in, yp, trow[k], tcol[k]);
+            }
+        }
     };
     // x-pass of one y-filtered plane, then ring slot U (slice s)
     auto xz_pass = [&](auto slot, int s, const float* yp) NDL_INLINE {
@@ -391,7 +404,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     auto publish = [&](const float (&v)[NLMAX], float* in) NDL_INLINE {
 #pragma unroll
         for (int k = 0; k < NLMAX; k++)
-            if (tid + k * NT < NIN) in[tid + k * NT] = v[k];
+            if (PAD || tid + k * NT < NIN) in[tid + k * NT] = v[k];
     };
 
     // Pipelined schedule, one barrier per step.  Step s: publish slice s+1

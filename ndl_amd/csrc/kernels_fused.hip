@@ -1,5 +1,6 @@
 // kernels_fused.hip -- forward instances of the fused separable kernel
 // (fused_tile.hpp) and the launch / occupancy dispatch.
+#include <algorithm>
 #include <cstdlib>
 
 #include "fused_tile.hpp"
@@ -92,7 +93,11 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
     const int NC = cfg.tx + (a.kx - 1) * cfg.il;
     const int NR = cfg.ty + a.ky - 1;
     const int NCP = (NC + 3 + 3) & ~3;
-    size_t floats = (size_t)2 * ((((NR + 3) * NC + 3) & ~3) + cfg.ty * NCP);
+    // the compile-time-tap instances pad the window to whole load slots
+    // (NLMAX * NT floats); a bound that covers every instance:
+    const int NT = cfg.tx * cfg.ty / cfg.cpt;
+    const size_t in_fl = std::max<size_t>((((NR + 3) * NC + 3) & ~3), (size_t)((NR * NC + NT - 1) / NT) * NT);
+    size_t floats = (size_t)2 * (in_fl + cfg.ty * NCP);
     // edge instances: weight rows of x, y and z (at most K rows per end)
     if (cfg.edge) floats += (size_t)2 * (a.kx * a.kx + a.ky * a.ky + a.kz * a.kz);
     floats += 64; // run-time x / y taps

@@ -9,6 +9,7 @@
 
 #include <ndl/convolution.h>
 #include <ndl/image.h>
+#include <ndl/morphology.h>
 
 using namespace ndl;
 
@@ -259,6 +260,67 @@ static void gradients()
     report("gradient() Reflect border at x=0", std::fabs(g(0, 0, 3) - 0.5 * (plane(1, 3) - plane(0, 3))) < 1e-12);
 }
 
+// image_morphology_tests.cpp:14-98
+static void image_morphology()
+{
+    std::vector<int> gridData(25);
+    Image<int, 2> grid(gridData.data(), {5, 5});
+    int i = 0;
+    for (auto it = grid.begin(); it != grid.end(); ++it) *it = ++i;
+    std::vector<double> boxData(9), crossData(9);
+    Image<double, 2> box(boxData.data(), {3, 3}), cross(crossData.data(), {3, 3});
+    make_box_kernel(box);
+    make_cross_kernel(cross);
+    std::vector<int> outData(25);
+    Image<int, 2> out(outData.data(), {5, 5});
+    erode(grid, out, box);
+    report("erode(box) takes the minimum of the 3x3 neighborhood", out(2, 2) == 7);
+    dilate(grid, out, box);
+    report("dilate(box) takes the maximum of the 3x3 neighborhood", out(2, 2) == 19);
+    erode(grid, out, cross);
+    report("erode(cross) considers only the 5-tap plus-sign neighborhood", out(2, 2) == 8);
+    dilate(grid, out, cross);
+    report("dilate(cross) considers only the 5-tap plus-sign neighborhood", out(2, 2) == 18);
+
+    std::vector<int> dotData(25, 0), dotOutData(25);
+    Image<int, 2> dot(dotData.data(), {5, 5}), dotOut(dotOutData.data(), {5, 5});
+    dot(2, 2) = 1;
+    dilate(dot, dotOut, box);
+    int ones = 0;
+    for (auto it = dotOut.begin(); it != dotOut.end(); ++it) ones += *it == 1;
+    report("dilate(box) of a single point turns on all 9 taps of the box", ones == 9);
+    dilate(dot, dotOut, cross);
+    ones = 0;
+    for (auto it = dotOut.begin(); it != dotOut.end(); ++it) ones += *it == 1;
+    report("dilate(cross) of a single point turns on only the 5 cross taps", ones == 5);
+
+    std::vector<int> stageData(25), manualData(25), openData(25), interData(25);
+    Image<int, 2> stage(stageData.data(), {5, 5}), manual(manualData.data(), {5, 5});
+    Image<int, 2> opened(openData.data(), {5, 5}), inter(interData.data(), {5, 5});
+    erode(grid, stage, box);
+    dilate(stage, manual, box);
+    morphological_open(grid, opened, inter, box);
+    report("morphological_open() matches erode() then dilate() done by hand", opened == manual);
+    dilate(grid, stage, box);
+    erode(stage, manual, box);
+    morphological_close(grid, opened, inter, box);
+    report("morphological_close() matches dilate() then erode() done by hand", opened == manual);
+
+    // 3-D box on a strided view equals the same call on its dense copy
+    std::mt19937 rng(7);
+    std::vector<float> vol(12 * 10 * 8);
+    for (auto& v : vol) v = std::uniform_real_distribution<float>(-1, 1)(rng);
+    Image<float, 3> V(vol.data(), {12, 10, 8});
+    auto view = V.view({1, 0, 1}, {11, 9, 7}, {2, 1, 1});
+    OwnedImage<float, 3> dense(view.extent()), a(view.extent()), b(view.extent());
+    dense.assign(view);
+    std::vector<double> k3(3 * 3 * 5, 1.0);
+    Image<double, 3> box3(k3.data(), {3, 3, 5});
+    dilate(view, a, box3, BorderMode::Reflect);
+    dilate(dense, b, box3, BorderMode::Reflect);
+    report("dilate() of a strided view equals dilate() of its dense copy", a == b);
+}
+
 int main()
 {
     int n = 0;
@@ -275,6 +337,7 @@ int main()
     per_axis_fp32();
     adjoints();
     gradients();
+    image_morphology();
     std::printf("%d failures\n", failures);
     return failures;
 }
