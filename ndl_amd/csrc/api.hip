@@ -391,8 +391,8 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     fa.sz = zd >= 0 ? src->stride[zd] : 0;
     fa.dz = zd >= 0 ? dst->stride[zd] : 0;
     if (src->extent[xd] > 0x7fffffff || fa.Y <= 0 || fa.Z <= 0) return false;
-    // in-slice offsets are 32-bit
-    if (std::llabs(fa.sy) * (int64_t)(fa.Y - 1) + fa.X >= 0x7fffffffLL) return false;
+    // in-slice offsets are 32-bit (loads: byte offsets)
+    if ((std::llabs(fa.sy) * (int64_t)(fa.Y - 1) + fa.X) * (int64_t)dtype_size(dt) >= 0xffffffffLL) return false;
     if (std::llabs(fa.dy) * (int64_t)(fa.Y - 1) + fa.X >= 0x7fffffffLL) return false;
     int64_t batch = 1;
     fa.nb = 0;

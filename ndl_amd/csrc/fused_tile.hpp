@@ -249,8 +249,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     const int tid = threadIdx.x;
 
     // per-thread load slots: element i = tid + k*NT of the window, border
-    // indices resolved once for the whole march
-    int32_t loff[NLMAX];
+    // indices resolved once for the whole march.  Offsets are taken from the
+    // smallest in-slice offset (row Y-1 when the row stride is negative) and
+    // kept in bytes, so they are non-negative 32-bit values added to a scalar
+    // base (the host keeps every slice span below 4 GiB).
+    const int64_t lmin = a.sy < 0 ? (int64_t)(a.Y - 1) * a.sy : 0;
+    uint32_t loff[NLMAX];
 #pragma unroll
     for (int k = 0; k < NLMAX; k++) {
         const int i = tid + k * NT;
@@ -268,15 +272,20 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
                 const int w = e >= 0 ? e / IL : -((-e + IL - 1) / IL);
                 xx = safe_map32(a.bx, a.X / IL, w) * IL + (e - w * IL);
             }
-            loff[k] = (int32_t)((int64_t)yy * a.sy + xx);
+            loff[k] = (uint32_t)(((int64_t)yy * a.sy + xx - lmin) * (int64_t)sizeof(TS));
         }
     }
-    // y-pass tasks: column c, rows [r0, r0+RY)
+    // y-pass tasks: column c, rows [r0, r0+RY); with one task per thread the
+    // tasks fill the first ceil(ntask / 64) waves
+    const int ywave = tid >> 6;
     int tcol[NTASKMAX], trow[NTASKMAX];
 #pragma unroll
     for (int k = 0; k < NTASKMAX; k++) {
         int task = tid + k * NT;
-        if (PAD && NTASKMAX == 1 && task >= NC * (TY / RY) && NC * (TY / RY) >= 64) task -= 64;
+        if constexpr (PAD && NTASKMAX == 1) {
+            task = ywave * 64 + (tid & 63);
+            if (task >= NC * (TY / RY) && NC * (TY / RY) >= 64) task -= 64;
+        }
         const int rg = task / NC;
         tcol[k] = task - rg * NC;
         trow[k] = rg * RY;
@@ -318,12 +327,16 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     float pre[NLMAX], pre2[NLMAX];
     // the z border map only matters for chunks that reach past an end
     const bool z_interior = z0 - a.cz >= 0 && z0 - a.cz + nsl <= a.Z;
+    const TS* src_l = src + lmin;                          // offsets are relative to this
+    const TS* src_z = src_l + (int64_t)(z0 - a.cz) * a.sz; // slice 0 of an interior chunk
     auto load_slice = [&](int s, float (&dstv)[NLMAX]) NDL_INLINE {
-        const int zz = z_interior ? z0 - a.cz + s : safe_map32(a.bz, a.Z, z0 - a.cz + s);
-        const TS* base = src + (int64_t)zz * a.sz;
+        const TS* base;
+        if (z_interior) base = src_z + (int64_t)s * a.sz;
+        else base = src_l + (int64_t)safe_map32(a.bz, a.Z, z0 - a.cz + s) * a.sz;
 #pragma unroll
         for (int k = 0; k < NLMAX; k++)
-            if (PAD || tid + k * NT < NIN) dstv[k] = load_s<TS>(base + loff[k]);
+            if (PAD || tid + k * NT < NIN)
+                dstv[k] = load_s<TS>((const TS*)((const unsigned char*)base + loff[k]));
     };
 
     float ring[CPT][P];
@@ -363,7 +376,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
         for (int k = 0; k < NTASKMAX; k++) {
             if constexpr (PAD && NTASKMAX == 1) {
                 // whole waves: the first ceil(ntask / 64) take a task each
-                if (__builtin_amdgcn_readfirstlane(tid >> 6) * 64 < ntask) y_task(in, yp, trow[k], tcol[k]);
+                if (__builtin_amdgcn_readfirstlane(ywave) * 64 < ntask) y_task(in, yp, trow[k], tcol[k]);
             } else if (tid + k * NT < ntask) {
                 y_task(in, yp, trow[k], tcol[k]);
             }
