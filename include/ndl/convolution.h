@@ -255,6 +255,28 @@ void convolve_separable(const SrcImageT& src, DstImageT& dst, const std::array<s
 /// the central difference {-0.5, 0, +0.5} along `axis` (the zero tap is
 /// skipped, as in convolve()), for every axis; dst has a leading component
 /// axis of size DIM.
+/// Block-mean shrink by `factor` along every axis but `channelAxis`
+/// (convolution.h:241-305); returns a new OwnedImage.  `border` is accepted
+/// for interface stability and, as in the reference, unused.
+template <class ImageT>
+auto downsample(const ImageT& src, int factor, int channelAxis = 0, BorderMode border = BorderMode::Clamp)
+{
+    (void)border;
+    using T = typename ImageT::value_type;
+    static_assert(std::is_arithmetic_v<T>, "ndl::downsample() requires an arithmetic value_type");
+    constexpr int DIM = detail::dim_of<ImageT>;
+    assert(factor > 1);
+    auto e = src.extent();
+    std::array<int, DIM> oe;
+    for (int i = 0; i < DIM; i++) oe[i] = i == channelAxis ? e[i] : (e[i] - 1) / factor + 1;
+    OwnedImage<T, DIM> result(oe);
+    detail::DeviceSource<ImageT> s(src);
+    detail::DeviceDest<OwnedImage<T, DIM>> o(result);
+    detail::check(ndl_downsample(&s.t, &o.t, factor, channelAxis, nullptr));
+    o.finish();
+    return result;
+}
+
 template <class SrcImageT, class DstImageT>
 void gradient(const SrcImageT& src, DstImageT& dst, BorderMode border = BorderMode::Reflect)
 {

@@ -149,6 +149,16 @@ int ndl_morph_separable(const ndl_tensor* src, const ndl_tensor* dst, int32_t na
 int ndl_morph_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* mask,
                  const int64_t* kextent, int32_t border, int32_t op, void* stream);
 
+/* ---- downsample ------------------------------------------------------------
+ * downsample(src, factor, channelAxis), convolution.h:241-305: every axis but
+ * channel_axis shrinks to (extent-1)/factor + 1; each output is the FP64 mean
+ * of its factor-sized source block clipped at the far edge (block order
+ * dimension 0 fastest, divided by the block size), std::round then cast for
+ * integral types.  dst must have that extent and src's element type; ndim >= 2.
+ * Bit-identical to the reference. */
+int ndl_downsample(const ndl_tensor* src, const ndl_tensor* dst, int32_t factor,
+                   int32_t channel_axis, void* stream);
+
 /* ---- adjoint (no counterpart in the reference) -------------------------
  * Gradient of L = <w, sepconv_forward(x)> with respect to x: grad_in =
  * A^T grad_out, the exact transpose of the forward operator including its

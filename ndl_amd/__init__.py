@@ -40,6 +40,7 @@ SIGNATURES = {
     "ndl_convolve_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, _VP]),
     "ndl_morph_separable": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
     "ndl_morph_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, C.c_int32, _VP]),
+    "ndl_downsample": (C.c_int, [_T, _T, C.c_int32, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_input": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_taps": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, _VP, C.c_int32, _VP]),
     "ndl_gaussian_taps": (C.c_int, [C.c_double, _DP, C.c_int32]),
@@ -239,6 +240,11 @@ def morph_nd_dev(src, dst, mask, border, op, stream=None):
                                     stream))
 
 
+def downsample_dev(src, dst, factor, channel_axis=0, stream=None):
+    ts, td = src.tensor(), dst.tensor()
+    return check(lib().ndl_downsample(C.byref(ts), C.byref(td), int(factor), int(channel_axis), stream))
+
+
 def separable_mask(mask):
     """Per-axis inclusion masks (numpy order) when the nonzero set of `mask`
     is their product, else None."""
@@ -301,6 +307,17 @@ def morph(img, mask, op, border="clamp", out_dtype=None, force_nd=False):
         morph_separable_dev(s, d, axes, masks, [border] * len(axes), op)
     else:
         morph_nd_dev(s, d, mask, border, op)
+    return d.to_numpy()
+
+
+def downsample(img, factor, channel_axis=0):
+    """ndl::downsample (convolution.h:241-305); channel_axis is an ndl axis
+    (0 = numpy's last)."""
+    n = img.ndim
+    shape = tuple(e if (n - 1 - i) == channel_axis else (e - 1) // factor + 1 for i, e in enumerate(img.shape))
+    s = DeviceArray.from_numpy(img)
+    d = DeviceArray(shape, img.dtype)
+    downsample_dev(s, d, factor, channel_axis)
     return d.to_numpy()
 
 

@@ -838,6 +838,47 @@ NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const 
     return NDL_OK;
 }
 
+// ---------------------------------------------------------------- downsample
+NDL_API int ndl_downsample(const ndl_tensor* src, const ndl_tensor* dst, int32_t factor, int32_t channel_axis,
+                           void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
+    if (src->dtype != dst->dtype) return fail(NDL_ERR_DTYPE, "src and dst element types differ");
+    if (src->ndim < 2 || dst->ndim != src->ndim) return fail(NDL_ERR_SHAPE, "downsample needs ndim >= 2 and equal ndim");
+    if (factor < 2) return fail(NDL_ERR_ARG, "factor %d < 2", factor);
+    if (channel_axis < 0 || channel_axis >= src->ndim)
+        return fail(NDL_ERR_ARG, "channel axis %d outside 0..%d", channel_axis, src->ndim - 1);
+    for (int d = 0; d < src->ndim; d++) {
+        if (src->extent[d] < 1) return fail(NDL_ERR_SHAPE, "empty source extent on dimension %d", d);
+        int64_t want = d == channel_axis ? src->extent[d] : (src->extent[d] - 1) / factor + 1;
+        if (dst->extent[d] != want)
+            return fail(NDL_ERR_SHAPE, "dst extent %lld on dimension %d, expected %lld", (long long)dst->extent[d], d,
+                        (long long)want);
+    }
+    if ((rc = check_no_overlap(src, dst))) return rc;
+    const int64_t n = numel(dst);
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 outputs in one call");
+    DownArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = src->data;
+    a.out = dst->data;
+    a.nd = src->ndim;
+    a.channel_axis = channel_axis;
+    a.factor = factor;
+    for (int d = 0; d < a.nd; d++) {
+        a.ext[d] = src->extent[d];
+        a.oext[d] = dst->extent[d];
+        a.ist[d] = src->stride[d];
+        a.ost[d] = dst->stride[d];
+        a.fdiv[d].init((uint32_t)dst->extent[d]);
+    }
+    a.total = (uint32_t)n;
+    hipError_t e = launch_downsample(a, src->dtype, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "downsample launch");
+    return NDL_OK;
+}
+
 // ---------------------------------------------------------------- morphology
 NDL_API int ndl_morph_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* mask, const int64_t* kextent,
                          int32_t border, int32_t op, void* stream)

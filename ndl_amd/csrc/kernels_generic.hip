@@ -292,6 +292,54 @@ hipError_t launch_morph_nd(const NdArgs& a, int dilate, int in_dt, int out_dt, h
     });
 }
 
+// ---------------------------------------------------------- downsample
+template <class T> __global__ __launch_bounds__(256) void downsample_kernel(const DownArgs a)
+{
+    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.total) return;
+    int64_t lo[NDL_MAX_DIM], hi[NDL_MAX_DIM], cur[NDL_MAX_DIM];
+    int64_t ooff = 0, count = 1;
+    uint32_t rem = idx;
+    for (int d = 0; d < a.nd; d++) {
+        uint32_t q = a.fdiv[d].div(rem);
+        const int64_t c = (int64_t)(rem - q * (uint32_t)a.oext[d]);
+        rem = q;
+        ooff += c * a.ost[d];
+        if (d == a.channel_axis) {
+            lo[d] = c;
+            hi[d] = c + 1;
+        } else {
+            lo[d] = c * a.factor;
+            hi[d] = min(lo[d] + a.factor, a.ext[d]);
+        }
+        cur[d] = lo[d];
+        count *= hi[d] - lo[d];
+    }
+    double sum = 0.0;
+    for (int64_t k = 0; k < count; k++) {
+        int64_t off = 0;
+        for (int d = 0; d < a.nd; d++) off += cur[d] * a.ist[d];
+        sum = sum + load_f64((const T*)a.in + off);
+        for (int d = 0; d < a.nd; d++) { // block odometer, dimension 0 fastest
+            if (d == a.channel_axis) continue;
+            if (++cur[d] < hi[d]) break;
+            cur[d] = lo[d];
+        }
+    }
+    store_from_f64((T*)a.out + ooff, sum / (double)count, NDL_ROUND_NEAREST);
+}
+
+hipError_t launch_downsample(const DownArgs& a, int dt, hipStream_t s)
+{
+    if (a.total == 0) return hipSuccess;
+    uint32_t blocks = (a.total + 255u) / 256u;
+    return with_dtype(dt, [&](auto t) {
+        using T = decltype(t);
+        hipLaunchKernelGGL((downsample_kernel<T>), dim3(blocks), dim3(256), 0, s, a);
+        return hipGetLastError();
+    });
+}
+
 // ------------------------------------------------------ tap gradients
 // Tap gradient of one axis: grad[t] = sum_p g[p] * U[p with its coordinate
 // j along the axis replaced by B(j + t - c)].  Work item = (segment of

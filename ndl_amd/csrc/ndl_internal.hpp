@@ -85,6 +85,23 @@ hipError_t launch_morph_pass(const MorphPassArgs& a, int in_dt, int out_dt, hipS
 // (NdArgs::tap_off lists the included taps; tap_val is unused)
 hipError_t launch_morph_nd(const NdArgs& a, int dilate, int in_dt, int out_dt, hipStream_t s);
 
+// --------------------------------------------------------- downsample
+// Block mean, processing/convolution.h:241-305: one output per thread,
+// FP64 sum over its clipped block in block order (dimension 0 fastest),
+// std::round into integral types.
+struct DownArgs {
+    const void* in;
+    void* out;
+    int32_t nd, channel_axis, factor;
+    int64_t ext[NDL_MAX_DIM];  // source extents
+    int64_t oext[NDL_MAX_DIM]; // output extents
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM]; // by the output extents, dimension 0 first
+    uint32_t total;
+};
+hipError_t launch_downsample(const DownArgs& a, int dt, hipStream_t s);
+
 // ------------------------------------------- tap-gradient partial sums
 // partial[blk*K + t] = sum over the block's share of positions p of
 // g[p] * u[B(p_axis + t - c)] (FP64), later summed in block order.

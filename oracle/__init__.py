@@ -41,6 +41,7 @@ def _load():
     lib = C.CDLL(LIB_PATH)
     abi.bind(lib, "oracle_convolve", C.c_int, _T, _T, _DP, _I64P, C.c_int)
     abi.bind(lib, "oracle_morph", C.c_int, _T, _T, _DP, _I64P, C.c_int, C.c_int)
+    abi.bind(lib, "oracle_downsample", C.c_int, _T, _T, C.c_int, C.c_int)
     abi.bind(lib, "oracle_sepconv", C.c_int, _T, _T, C.c_int, _I32P, _K, C.c_int)
     abi.bind(lib, "oracle_sepconv_adjoint_input", C.c_int, _T, _T, C.c_int, _I32P, _K)
     abi.bind(lib, "oracle_sepconv_adjoint_taps", C.c_int, _T, _T, C.c_int, _I32P, _K, _DP)
@@ -64,6 +65,8 @@ def _load_ref():
     abi.bind(lib, "ref_kernel_size", C.c_int, C.c_double)
     if hasattr(lib, "ref_morph"):
         abi.bind(lib, "ref_morph", C.c_int, _T, _T, _DP, _I64P, C.c_int, C.c_int)
+    if hasattr(lib, "ref_downsample"):
+        abi.bind(lib, "ref_downsample", C.c_int, _T, _T, C.c_int, C.c_int)
     return lib
 
 
@@ -133,6 +136,25 @@ def morph(src, kernel, op, border="clamp", dst=None, use_ref=False):
         _check(ref_lib.ref_morph(C.byref(ts), C.byref(td), kp, kext, _border(border), o), "ref_morph")
     else:
         _check(lib.oracle_morph(C.byref(ts), C.byref(td), kp, kext, _border(border), o), "oracle_morph")
+    return dst
+
+
+def downsample_shape(shape, factor, channel_axis=0):
+    """Result shape (numpy order) of ndl::downsample; channel_axis is an ndl
+    axis (0 = numpy's last)."""
+    n = len(shape)
+    return tuple(e if (n - 1 - i) == channel_axis else (e - 1) // factor + 1 for i, e in enumerate(shape))
+
+
+def downsample(src, factor, channel_axis=0, use_ref=False):
+    """ndl::downsample(src, factor, channelAxis), convolution.h:241-305."""
+    src = np.ascontiguousarray(src)
+    dst = np.zeros(downsample_shape(src.shape, factor, channel_axis), dtype=src.dtype)
+    ts, td = abi.host_tensor(src), abi.host_tensor(dst)
+    if use_ref:
+        _check(ref_lib.ref_downsample(C.byref(ts), C.byref(td), factor, channel_axis), "ref_downsample")
+    else:
+        _check(lib.oracle_downsample(C.byref(ts), C.byref(td), factor, channel_axis), "oracle_downsample")
     return dst
 
 

@@ -24,6 +24,12 @@
  *                           reference's compiled code does)
  *   oracle_sepconv_adjoint_input / _taps: no reference counterpart; the exact
  *                           transpose / tap derivative of oracle_sepconv in FP64.
+ *   oracle_downsample       processing/convolution.h:241-305 (block mean:
+ *                           every axis but channelAxis shrinks to
+ *                           (e-1)/factor+1; each output sums its clipped
+ *                           source block in FP64, block coordinates
+ *                           dimension 0 fastest, divides by the block size,
+ *                           std::round for integral types, static_cast)
  *   oracle_morph            processing/morphology.h:57-101 (erode / dilate:
  *                           first included tap in kernel-coordinate order,
  *                           then strictly smaller / larger values replace it;
@@ -513,6 +519,52 @@ EXPORT int oracle_morph(const ndl_tensor* src, const ndl_tensor* dst, const doub
         }
     }
     free(tapc);
+    return NDL_OK;
+}
+
+EXPORT int oracle_downsample(const ndl_tensor* src, const ndl_tensor* dst, int factor, int channel_axis)
+{
+    int nd = src->ndim;
+    if (nd < 2 || nd > NDL_MAX_DIM || dst->ndim != nd || factor < 2 || channel_axis < 0 || channel_axis >= nd)
+        return NDL_ERR_ARG;
+    for (int d = 0; d < nd; d++) {
+        int64_t want = d == channel_axis ? src->extent[d] : (src->extent[d] - 1) / factor + 1;
+        if (src->extent[d] < 1 || dst->extent[d] != want) return NDL_ERR_SHAPE;
+    }
+    int64_t n = numel(dst), coord[NDL_MAX_DIM] = {0};
+    for (int64_t lin = 0; lin < n; lin++) {
+        int64_t lo[NDL_MAX_DIM], hi[NDL_MAX_DIM], cur[NDL_MAX_DIM], count = 1;
+        for (int d = 0; d < nd; d++) {
+            if (d == channel_axis) {
+                lo[d] = coord[d];
+                hi[d] = coord[d] + 1;
+            } else {
+                lo[d] = coord[d] * factor;
+                hi[d] = lo[d] + factor < src->extent[d] ? lo[d] + factor : src->extent[d];
+            }
+            count *= hi[d] - lo[d];
+            cur[d] = lo[d];
+        }
+        double sum = 0;
+        for (int64_t k = 0; k < count; k++) {
+            int64_t off = 0;
+            for (int d = 0; d < nd; d++) off += cur[d] * src->stride[d];
+            sum += load_elem(src->data, src->dtype, off);
+            for (int d = 0; d < nd; d++) { /* block odometer, dimension 0 fastest */
+                if (d == channel_axis) continue;
+                if (++cur[d] < hi[d]) break;
+                cur[d] = lo[d];
+            }
+        }
+        double avg = sum / (double)count;
+        int64_t doff = 0;
+        for (int d = 0; d < nd; d++) doff += coord[d] * dst->stride[d];
+        store_elem(dst->data, dst->dtype, doff, avg, NDL_ROUND_NEAREST);
+        for (int d = 0; d < nd; d++) {
+            if (++coord[d] < dst->extent[d]) break;
+            coord[d] = 0;
+        }
+    }
     return NDL_OK;
 }
 

@@ -9,7 +9,8 @@
 // (processing/convolution.h:52-71,107-186) on dense Image<T,DIM> views
 // (image/core.h:228-235) and composes per-axis passes the way a caller of the
 // reference would (double intermediates, as convolution.h:151-185 does);
-// ndl::erode / ndl::dilate (processing/morphology.h:57-101) likewise.
+// ndl::erode / ndl::dilate (processing/morphology.h:57-101) and
+// ndl::downsample (processing/convolution.h:241-305) likewise.
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -169,6 +170,30 @@ int dispatch_dim_morph(const ndl_tensor* s, const ndl_tensor* d, const double* k
     return NDL_ERR_SHAPE;
 }
 
+template <class T, int DIM>
+int downsample_t(const ndl_tensor* src, const ndl_tensor* dst, int factor, int channel_axis)
+{
+    ndl::Image<T, DIM> s((T*)src->data, ext_of<DIM>(src));
+    auto r = ndl::downsample(s, factor, channel_axis);
+    auto e = r.extent();
+    for (int d = 0; d < DIM; d++)
+        if (e[d] != dst->extent[d]) return NDL_ERR_SHAPE;
+    ndl::Image<T, DIM> d((T*)dst->data, e);
+    for (const auto& c : r.coordinates()) d.at(c) = r.at(c);
+    return 0;
+}
+
+template <class T>
+int dispatch_dim_down(const ndl_tensor* s, const ndl_tensor* d, int factor, int channel_axis)
+{
+    switch (s->ndim) {
+    case 2: return downsample_t<T, 2>(s, d, factor, channel_axis);
+    case 3: return downsample_t<T, 3>(s, d, factor, channel_axis);
+    case 4: return downsample_t<T, 4>(s, d, factor, channel_axis);
+    }
+    return NDL_ERR_SHAPE;
+}
+
 bool dense_same(const ndl_tensor* s, const ndl_tensor* d)
 {
     if (s->dtype != d->dtype || s->ndim != d->ndim) return false;
@@ -230,6 +255,14 @@ __attribute__((visibility("default"))) int ref_morph(const ndl_tensor* src, cons
 {
     if (!dense_same(src, dst) || (op != 0 && op != 1)) return NDL_ERR_ARG;
     NDL_REF_DISPATCH(dispatch_dim_morph, src, dst, kernel, kextent, border, op)
+}
+
+// downsample() into a dense dst of the result's extent (src dense)
+__attribute__((visibility("default"))) int ref_downsample(const ndl_tensor* src, const ndl_tensor* dst, int factor,
+                                                          int channel_axis)
+{
+    if (src->dtype != dst->dtype || factor < 2) return NDL_ERR_ARG;
+    NDL_REF_DISPATCH(dispatch_dim_down, src, dst, factor, channel_axis)
 }
 
 // mathHelpers.h _kernelSize

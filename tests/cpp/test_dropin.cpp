@@ -321,6 +321,31 @@ static void image_morphology()
     report("dilate() of a strided view equals dilate() of its dense copy", a == b);
 }
 
+// image_convolution_tests.cpp:70-91
+static void downsample_tests()
+{
+    OwnedImage<uint8_t, 3> constColor({3, 40, 30});
+    constColor = uint8_t(77);
+    auto constDown = downsample(constColor, 2);
+    report("downsample() extent: channel axis untouched, spatial axes halved",
+           constDown.extent() == std::array<int, 3>{3, 20, 15});
+    bool constMatches = true;
+    for (const auto& c : constDown.coordinates())
+        if (constDown.at(c) != 77) constMatches = false;
+    report("downsample() of a constant image stays constant", constMatches);
+    auto explicitAxis = downsample(constColor, 2, 0, BorderMode::Clamp);
+    report("downsample() default channelAxis matches explicit channelAxis=0", explicitAxis == constDown);
+    // a ramp: 2x2 block means of 0..(w*h-1) are exact halves
+    OwnedImage<float, 2> ramp({8, 6});
+    int i = 0;
+    for (auto it = ramp.begin(); it != ramp.end(); ++it) *it = (float)i++;
+    auto half = downsample(ramp, 2, 1);
+    bool ok = half.extent() == std::array<int, 2>{4, 6};
+    for (int y = 0; y < 6 && ok; y++)
+        for (int x = 0; x < 4; x++) ok = ok && half(x, y) == (float)(y * 8 + 2 * x) + 0.5f;
+    report("downsample() along x only (channelAxis 1) averages pairs", ok);
+}
+
 int main()
 {
     int n = 0;
@@ -338,6 +363,7 @@ int main()
     adjoints();
     gradients();
     image_morphology();
+    downsample_tests();
     std::printf("%d failures\n", failures);
     return failures;
 }

@@ -133,3 +133,40 @@ def test_open_close_idempotent():
     assert np.all(opened <= x) and np.all(closed >= x)
     assert np.array_equal(ndl_amd.morph(ndl_amd.morph(opened, k, "erode"), k, "dilate"), opened)
     assert np.array_equal(ndl_amd.morph(ndl_amd.morph(closed, k, "dilate"), k, "erode"), closed)
+
+
+# ------------------------------------------------------------- downsample
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16, np.int32, np.float32, np.float64,
+                                   np.float16])
+def test_downsample_vs_oracle(dtype):
+    rng = np.random.default_rng(53)
+    for shape, f, ca in [((30, 40, 3), 2, 0), ((13, 17), 3, 1), ((5, 9, 11), 2, 0), ((3, 20, 21), 4, 2),
+                         ((6, 7, 8, 9), 3, 1), ((2, 31), 5, 1), ((256, 257), 2, 1)]:
+        if np.issubdtype(dtype, np.floating):
+            x = (rng.standard_normal(shape) * 100).astype(dtype)
+        else:
+            info = np.iinfo(dtype)
+            x = rng.integers(info.min, info.max, size=shape, endpoint=True).astype(dtype)
+        say("downsample", np.dtype(dtype).name, shape, f, ca)
+        want = oracle.downsample(x, f, ca)
+        got = ndl_amd.downsample(x, f, ca)
+        assert same_bits(got, want), (np.dtype(dtype).name, shape, f, ca)
+
+
+def test_downsample_reference_constant_colour():
+    # image_convolution_tests.cpp:70-91
+    x = np.full((30, 40, 3), 77, dtype=np.uint8)
+    out = ndl_amd.downsample(x, 2)
+    assert out.shape == (15, 20, 3) and np.all(out == 77)
+
+
+def test_downsample_strided_source():
+    rng = np.random.default_rng(54)
+    x = rng.standard_normal((40, 50)).astype(np.float32)
+    d = ndl_amd.DeviceArray.from_numpy(x)
+    src = d.view((slice(None, None, -1), slice(3, 48, 2)))
+    dense = np.ascontiguousarray(x[::-1, 3:48:2])
+    want = oracle.downsample(dense, 3, 1)
+    out = ndl_amd.DeviceArray(want.shape, np.float32)
+    ndl_amd.downsample_dev(src, out, 3, 1)
+    assert same_bits(out.to_numpy(), want)
