@@ -26,6 +26,7 @@
 #include <cassert>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
@@ -255,6 +256,55 @@ void convolve_separable(const SrcImageT& src, DstImageT& dst, const std::array<s
 /// the central difference {-0.5, 0, +0.5} along `axis` (the zero tap is
 /// skipped, as in convolve()), for every axis; dst has a leading component
 /// axis of size DIM.
+/// gaussian_blur() of one image at several sigmas: levels[l] =
+/// gaussian_blur(src, sigma[l], border), bit-identical to separate calls
+/// (the scale stack detect_keypoints builds, feature_detection.h:101-109);
+/// the first axis pass of every level reads src once.
+template <class SrcImageT, class LevelImageT>
+void gaussian_blur_scales(const SrcImageT& src, std::vector<LevelImageT>& levels, const std::vector<double>& sigma,
+                          BorderMode border = BorderMode::Clamp)
+{
+    using DstT = typename LevelImageT::value_type;
+    constexpr int DIM = detail::dim_of<SrcImageT>;
+    assert(levels.size() == sigma.size());
+    for (double sg : sigma) assert(sg > 0);
+    (void)sizeof(DstT);
+    if (levels.empty()) return;
+    detail::DeviceSource<SrcImageT> s(src);
+    std::vector<std::unique_ptr<detail::DeviceDest<LevelImageT>>> o;
+    std::vector<ndl_tensor> t;
+    for (auto& lv : levels) {
+        assert(lv.extent() == src.extent());
+        o.emplace_back(new detail::DeviceDest<LevelImageT>(lv));
+        t.push_back(o.back()->t);
+    }
+    const int32_t rounding = std::is_integral_v<DstT> && DIM >= 2 ? NDL_ROUND_NEAREST : NDL_ROUND_TRUNC;
+    detail::check(ndl_gaussian_scales(&s.t, t.data(), (int32_t)t.size(), sigma.data(), detail::border_code(border),
+                                      NDL_ACCUM_REF, rounding, nullptr));
+    for (auto& d : o) d->finish();
+}
+
+/// Difference-of-Gaussians layers dogs[l] = levels[l+1] - levels[l]
+/// (feature_detection.h:111-120), in FP64, stored as DogImageT's type.
+template <class LevelImageT, class DogImageT>
+void difference_of_gaussians(const std::vector<LevelImageT>& levels, std::vector<DogImageT>& dogs)
+{
+    assert(levels.size() >= 2 && dogs.size() + 1 == levels.size());
+    std::vector<std::unique_ptr<detail::DeviceSource<LevelImageT>>> in;
+    std::vector<std::unique_ptr<detail::DeviceDest<DogImageT>>> out;
+    std::vector<ndl_tensor> lt, dt;
+    for (const auto& lv : levels) {
+        in.emplace_back(new detail::DeviceSource<LevelImageT>(lv));
+        lt.push_back(in.back()->t);
+    }
+    for (auto& d : dogs) {
+        out.emplace_back(new detail::DeviceDest<DogImageT>(d));
+        dt.push_back(out.back()->t);
+    }
+    detail::check(ndl_dog(lt.data(), dt.data(), (int32_t)lt.size(), nullptr));
+    for (auto& d : out) d->finish();
+}
+
 /// Block-mean shrink by `factor` along every axis but `channelAxis`
 /// (convolution.h:241-305); returns a new OwnedImage.  `border` is accepted
 /// for interface stability and, as in the reference, unused.

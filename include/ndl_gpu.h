@@ -149,6 +149,25 @@ int ndl_morph_separable(const ndl_tensor* src, const ndl_tensor* dst, int32_t na
 int ndl_morph_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* mask,
                  const int64_t* kextent, int32_t border, int32_t op, void* stream);
 
+/* ---- scale stacks ------------------------------------------------------------
+ * Several gaussian_blur() levels of one image: levels[l] = gaussian_blur(src,
+ * sigmas[l], border) over every axis (convolution.h:107-186), the scale stack
+ * of detect_keypoints (feature_detection.h:101-109).  accum / rounding as in
+ * ndl_sepconv_forward (the C++ layer passes NDL_ACCUM_REF and, for integral
+ * images with ndim >= 2, NDL_ROUND_NEAREST: bit-identical to the reference).
+ * In NDL_ACCUM_REF the first axis pass of every level reads src once.
+ * nlevels <= 16; each level like a dst of ndl_sepconv_forward; levels must not
+ * overlap src or each other. */
+int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels, int32_t nlevels,
+                        const double* sigmas, int32_t border, int32_t accum, int32_t rounding,
+                        void* stream);
+
+/* Difference-of-Gaussians layers (feature_detection.h:111-120):
+ * dogs[l] = levels[l+1] - levels[l] for l < nlevels-1, computed in FP64 and
+ * stored as dogs' element type (exact for float64 levels, as the reference's
+ * OwnedImage<double> layers).  Equal extents, no overlap with the levels. */
+int ndl_dog(const ndl_tensor* levels, const ndl_tensor* dogs, int32_t nlevels, void* stream);
+
 /* ---- downsample ------------------------------------------------------------
  * downsample(src, factor, channelAxis), convolution.h:241-305: every axis but
  * channel_axis shrinks to (extent-1)/factor + 1; each output is the FP64 mean

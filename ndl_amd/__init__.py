@@ -41,6 +41,8 @@ SIGNATURES = {
     "ndl_morph_separable": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
     "ndl_morph_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, C.c_int32, _VP]),
     "ndl_downsample": (C.c_int, [_T, _T, C.c_int32, C.c_int32, _VP]),
+    "ndl_gaussian_scales": (C.c_int, [_T, _T, C.c_int32, _DP, C.c_int32, C.c_int32, C.c_int32, _VP]),
+    "ndl_dog": (C.c_int, [_T, _T, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_input": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_taps": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, _VP, C.c_int32, _VP]),
     "ndl_gaussian_taps": (C.c_int, [C.c_double, _DP, C.c_int32]),
@@ -238,6 +240,24 @@ def morph_nd_dev(src, dst, mask, border, op, stream=None):
     ts, td = src.tensor(), dst.tensor()
     return check(lib().ndl_morph_nd(C.byref(ts), C.byref(td), mask.ctypes.data_as(_DP), kext, b, MORPH_OPS[op],
                                     stream))
+
+
+def gaussian_scales_dev(src, levels, sigmas, border="clamp", accum=abi.ACCUM_REF, rounding=abi.ROUND_TRUNC,
+                        stream=None):
+    """levels[l] = gaussian_blur(src, sigmas[l], border) for DeviceArrays."""
+    arr = (abi.Tensor * len(levels))(*[lv.tensor() for lv in levels])
+    sg = np.ascontiguousarray(sigmas, dtype=np.float64)
+    b = abi.BORDERS[border] if isinstance(border, str) else int(border)
+    ts = src.tensor()
+    return check(lib().ndl_gaussian_scales(C.byref(ts), arr, len(levels), sg.ctypes.data_as(_DP), b, accum,
+                                           rounding, stream))
+
+
+def dog_dev(levels, dogs, stream=None):
+    """dogs[l] = levels[l+1] - levels[l]."""
+    la = (abi.Tensor * len(levels))(*[lv.tensor() for lv in levels])
+    da = (abi.Tensor * max(1, len(dogs)))(*[d.tensor() for d in dogs])
+    return check(lib().ndl_dog(la, da, len(levels), stream))
 
 
 def downsample_dev(src, dst, factor, channel_axis=0, stream=None):

@@ -1061,6 +1061,130 @@ NDL_API int ndl_gaussian_taps(double sigma, double* taps, int32_t cap)
     return width;
 }
 
+// ---------------------------------------------------------- scale stacks
+NDL_API int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels, int32_t nlevels, const double* sigmas,
+                                int32_t border, int32_t accum, int32_t rounding, void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(src, "src"))) return rc;
+    if (nlevels < 1 || nlevels > NDL_MAX_LEVELS) return fail(NDL_ERR_ARG, "nlevels %d outside 1..%d", nlevels, NDL_MAX_LEVELS);
+    if (!levels || !sigmas) return fail(NDL_ERR_ARG, "null levels or sigmas");
+    if (border < NDL_CLAMP || border > NDL_REFLECT) return fail(NDL_ERR_ARG, "unknown border %d", border);
+    const int nd = src->ndim;
+    std::vector<std::vector<double>> taps(nlevels, std::vector<double>(NDL_MAX_TAPS));
+    std::vector<std::vector<ndl_axis_kernel>> k(nlevels);
+    int32_t axes[NDL_MAX_DIM];
+    for (int d = 0; d < nd; d++) axes[d] = d;
+    for (int l = 0; l < nlevels; l++) {
+        int w = ndl_gaussian_taps(sigmas[l], taps[l].data(), NDL_MAX_TAPS);
+        if (w < 0) return fail(w, "level %d: %s", l, g_err.c_str());
+        for (int d = 0; d < nd; d++) k[l].push_back(ndl_axis_kernel{taps[l].data(), w, w / 2, border});
+        if ((rc = check_forward_args(src, &levels[l], nd, axes, k[l].data(), accum, rounding)))
+            return fail(rc, "level %d: %s", l, g_err.c_str());
+        for (int m = 0; m < l; m++)
+            if ((rc = check_no_overlap(&levels[m], &levels[l]))) return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = numel(src);
+    if (n == 0) return NDL_OK;
+    if (accum == NDL_ACCUM_F32 || nd == 1) {
+        // one launch per level (the fused kernel is compute-bound, so reading
+        // the source once for all levels would not shorten it)
+        for (int l = 0; l < nlevels; l++)
+            if ((rc = forward_impl(src, &levels[l], nd, axes, k[l].data(), accum, rounding, s))) return rc;
+        return NDL_OK;
+    }
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    // exact mode: the axis-0 pass of every level from one read of the source
+    // (gaussian_blur's first FP64 buffer, convolution.h:151-169), then each
+    // level's remaining axes through the FP64 chain
+    std::vector<Scratch> first(nlevels);
+    std::vector<ndl_tensor> mid(nlevels);
+    for (int l = 0; l < nlevels; l++) {
+        if ((rc = alloc_scratch(first[l], (size_t)n * sizeof(double), s))) return rc;
+        mid[l] = dense_like(src, first[l].p, NDL_F64);
+    }
+    Scratch tb;
+    if ((rc = alloc_scratch(tb, (size_t)nlevels * NDL_MAX_TAPS * sizeof(double), s))) return rc;
+    std::vector<double> host((size_t)nlevels * NDL_MAX_TAPS, 0.0);
+    for (int l = 0; l < nlevels; l++) std::copy(taps[l].begin(), taps[l].end(), host.begin() + (size_t)l * NDL_MAX_TAPS);
+    hipError_t e = hipMemcpyAsync(tb.p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s); // the host copy dies with this call
+    if (e != hipSuccess) return hip_fail(e, "scale taps upload");
+    MultiAxisArgs ma;
+    memset(&ma, 0, sizeof ma);
+    AxisPassArgs& p = ma.p;
+    p.in = src->data;
+    p.m = src->extent[0];
+    p.in_as = src->stride[0];
+    int order[NDL_MAX_DIM];
+    for (int d = 0; d < nd; d++) order[d] = d;
+    std::stable_sort(order, order + nd, [&](int x, int y) { return std::llabs(src->stride[x]) < std::llabs(src->stride[y]); });
+    p.nd = nd;
+    for (int i = 0; i < nd; i++) {
+        int d = order[i];
+        p.ext[i] = src->extent[d];
+        p.ist[i] = src->stride[d];
+        p.ost[i] = mid[0].stride[d];
+        p.fdiv[i].init((uint32_t)src->extent[d]);
+        if (d == 0) p.axis_pos = i;
+    }
+    p.total = (uint32_t)n;
+    p.border = border;
+    ma.nlev = nlevels;
+    ma.taps = (const double*)tb.p;
+    for (int l = 0; l < nlevels; l++) {
+        ma.outs[l] = first[l].p;
+        ma.K[l] = k[l][0].count;
+        ma.center[l] = k[l][0].center;
+    }
+    e = launch_axis_pass_multi(ma, src->dtype, s);
+    if (e != hipSuccess) return hip_fail(e, "scale-stack pass launch");
+    for (int l = 0; l < nlevels; l++)
+        if ((rc = run_generic_chain(&mid[l], &levels[l], nd - 1, axes + 1, k[l].data() + 1, accum, rounding, s)))
+            return rc;
+    return NDL_OK;
+}
+
+NDL_API int ndl_dog(const ndl_tensor* levels, const ndl_tensor* dogs, int32_t nlevels, void* stream)
+{
+    int rc;
+    if (nlevels < 2 || !levels || !dogs) return fail(NDL_ERR_ARG, "need at least two levels");
+    for (int l = 0; l < nlevels; l++) {
+        if ((rc = check_tensor(&levels[l], "level"))) return rc;
+        if ((rc = check_same_shape(&levels[0], &levels[l], "level 0", "level"))) return rc;
+        if (levels[l].dtype != levels[0].dtype) return fail(NDL_ERR_DTYPE, "levels differ in element type");
+    }
+    for (int l = 0; l + 1 < nlevels; l++) {
+        if ((rc = check_tensor(&dogs[l], "dog"))) return rc;
+        if ((rc = check_same_shape(&levels[0], &dogs[l], "level 0", "dog"))) return rc;
+        for (int m = 0; m < nlevels; m++)
+            if ((rc = check_no_overlap(&levels[m], &dogs[l]))) return rc;
+    }
+    const int64_t n = numel(&levels[0]);
+    if (n == 0) return NDL_OK;
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    for (int l = 0; l + 1 < nlevels; l++) {
+        DiffArgs a;
+        memset(&a, 0, sizeof a);
+        a.a = levels[l].data;
+        a.b = levels[l + 1].data;
+        a.out = dogs[l].data;
+        a.nd = levels[0].ndim;
+        for (int d = 0; d < a.nd; d++) {
+            a.ext[d] = levels[0].extent[d];
+            a.ast[d] = levels[l].stride[d];
+            a.bst[d] = levels[l + 1].stride[d];
+            a.ost[d] = dogs[l].stride[d];
+            a.fdiv[d].init((uint32_t)levels[0].extent[d]);
+        }
+        a.total = (uint32_t)n;
+        hipError_t e = launch_difference(a, levels[0].dtype, dogs[l].dtype, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "difference launch");
+    }
+    return NDL_OK;
+}
+
 // ---------------------------------------------------------------- helpers
 NDL_API int ndl_malloc(void** ptr, size_t bytes)
 {

@@ -292,6 +292,79 @@ hipError_t launch_morph_nd(const NdArgs& a, int dilate, int in_dt, int out_dt, h
     });
 }
 
+// ------------------------------------------------------- scale stacks
+template <class TIn> __global__ __launch_bounds__(256) void axis_pass_multi_kernel(const MultiAxisArgs a)
+{
+    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.p.total) return;
+    int64_t ioff = 0, ooff = 0, j = 0;
+    uint32_t rem = idx;
+    for (int d = 0; d < a.p.nd; d++) {
+        uint32_t q = a.p.fdiv[d].div(rem);
+        int64_t c = (int64_t)(rem - q * (uint32_t)a.p.ext[d]);
+        rem = q;
+        ioff += c * a.p.ist[d];
+        ooff += c * a.p.ost[d];
+        if (d == a.p.axis_pos) j = c;
+    }
+    const TIn* in = (const TIn*)a.p.in + (ioff - j * a.p.in_as);
+    for (int l = 0; l < a.nlev; l++) {
+        const double* w = a.taps + (size_t)l * NDL_MAX_TAPS;
+        double total = 0;
+        for (int t = 0; t < a.K[l]; t++) {
+            const double wt = w[t];
+            if (wt == 0.0) continue;
+            int64_t x = j + t - a.center[l];
+            if (x < 0 || x >= a.p.m) x = border_map(a.p.border, a.p.m, x);
+            total = total + load_f64(in + x * a.p.in_as) * wt; // the level's line stays in cache
+        }
+        ((double*)a.outs[l])[ooff] = total;
+    }
+}
+
+hipError_t launch_axis_pass_multi(const MultiAxisArgs& a, int in_dt, hipStream_t s)
+{
+    if (a.p.total == 0) return hipSuccess;
+    uint32_t blocks = (a.p.total + 255u) / 256u;
+    return with_dtype(in_dt, [&](auto tin) {
+        using TIn = decltype(tin);
+        hipLaunchKernelGGL((axis_pass_multi_kernel<TIn>), dim3(blocks), dim3(256), 0, s, a);
+        return hipGetLastError();
+    });
+}
+
+template <class TIn, class TOut> __global__ __launch_bounds__(256) void difference_kernel(const DiffArgs a)
+{
+    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.total) return;
+    int64_t ao = 0, bo = 0, oo = 0;
+    uint32_t rem = idx;
+    for (int d = 0; d < a.nd; d++) {
+        uint32_t q = a.fdiv[d].div(rem);
+        int64_t c = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+        rem = q;
+        ao += c * a.ast[d];
+        bo += c * a.bst[d];
+        oo += c * a.ost[d];
+    }
+    store_from_f64((TOut*)a.out + oo, load_f64((const TIn*)a.b + bo) - load_f64((const TIn*)a.a + ao),
+                   NDL_ROUND_TRUNC);
+}
+
+hipError_t launch_difference(const DiffArgs& a, int in_dt, int out_dt, hipStream_t s)
+{
+    if (a.total == 0) return hipSuccess;
+    uint32_t blocks = (a.total + 255u) / 256u;
+    return with_dtype(in_dt, [&](auto tin) {
+        using TIn = decltype(tin);
+        return with_dtype(out_dt, [&](auto tout) {
+            using TOut = decltype(tout);
+            hipLaunchKernelGGL((difference_kernel<TIn, TOut>), dim3(blocks), dim3(256), 0, s, a);
+            return hipGetLastError();
+        });
+    });
+}
+
 // ---------------------------------------------------------- downsample
 template <class T> __global__ __launch_bounds__(256) void downsample_kernel(const DownArgs a)
 {

@@ -85,6 +85,32 @@ hipError_t launch_morph_pass(const MorphPassArgs& a, int in_dt, int out_dt, hipS
 // (NdArgs::tap_off lists the included taps; tap_val is unused)
 hipError_t launch_morph_nd(const NdArgs& a, int dilate, int in_dt, int out_dt, hipStream_t s);
 
+// ------------------------------------------------------ scale stacks
+// First pass of several gaussian_blur() levels of one image: along one axis,
+// every level's sum from the same input line (FP64, separately rounded,
+// taps ascending, zero taps skipped), one dense FP64 output per level.
+constexpr int NDL_MAX_LEVELS = 16;
+struct MultiAxisArgs {
+    AxisPassArgs p;                 // input, axis, iteration order, border (taps unused)
+    int32_t nlev;
+    void* outs[NDL_MAX_LEVELS];     // dense FP64, strides p.ost
+    int32_t K[NDL_MAX_LEVELS], center[NDL_MAX_LEVELS];
+    const double* taps;             // device, nlev x NDL_MAX_TAPS
+};
+hipError_t launch_axis_pass_multi(const MultiAxisArgs& a, int in_dt, hipStream_t s);
+// out = b - a elementwise, in FP64, stored as out's type (dense or strided)
+struct DiffArgs {
+    const void* a;
+    const void* b;
+    void* out;
+    int32_t nd;
+    int64_t ext[NDL_MAX_DIM];
+    int64_t ast[NDL_MAX_DIM], bst[NDL_MAX_DIM], ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t total;
+};
+hipError_t launch_difference(const DiffArgs& a, int in_dt, int out_dt, hipStream_t s);
+
 // --------------------------------------------------------- downsample
 // Block mean, processing/convolution.h:241-305: one output per thread,
 // FP64 sum over its clipped block in block order (dimension 0 fastest),

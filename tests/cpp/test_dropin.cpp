@@ -346,6 +346,47 @@ static void downsample_tests()
     report("downsample() along x only (channelAxis 1) averages pairs", ok);
 }
 
+// the scale stack and DoG layers of detect_keypoints (feature_detection.h:91-120)
+static void scale_stack()
+{
+    std::mt19937 rng(11);
+    OwnedImage<double, 2> src({70, 50});
+    for (auto it = src.begin(); it != src.end(); ++it) *it = std::uniform_real_distribution<double>(0, 255)(rng);
+    const int numScales = 5;
+    const double sigma0 = 1.6;
+    std::vector<double> sig;
+    std::vector<OwnedImage<double, 2>> levels, singles, dogs;
+    for (int i = 0; i < numScales; i++) {
+        sig.push_back(sigma0 * std::pow(2.0, (double)i / (numScales - 3)));
+        levels.emplace_back(src.extent());
+        singles.emplace_back(src.extent());
+        gaussian_blur(src, singles.back(), sig.back(), BorderMode::Reflect);
+    }
+    gaussian_blur_scales(src, levels, sig, BorderMode::Reflect);
+    bool same = true;
+    for (int i = 0; i < numScales; i++) same = same && levels[i] == singles[i];
+    report("gaussian_blur_scales() equals one gaussian_blur() per sigma", same);
+    for (int i = 0; i + 1 < numScales; i++) dogs.emplace_back(src.extent());
+    difference_of_gaussians(levels, dogs);
+    bool ok = true;
+    for (int i = 0; i + 1 < numScales; i++)
+        for (const auto& c : dogs[i].coordinates()) ok = ok && dogs[i].at(c) == levels[i + 1].at(c) - levels[i].at(c);
+    report("difference_of_gaussians() layers are level[i+1] - level[i] in double", ok);
+    // integral images round like gaussian_blur (std::round, DIM >= 2)
+    OwnedImage<uint8_t, 2> u8({64, 48});
+    for (auto it = u8.begin(); it != u8.end(); ++it) *it = (uint8_t)std::uniform_int_distribution<int>(0, 255)(rng);
+    std::vector<OwnedImage<uint8_t, 2>> ul, us;
+    for (int i = 0; i < 3; i++) {
+        ul.emplace_back(u8.extent());
+        us.emplace_back(u8.extent());
+        gaussian_blur(u8, us.back(), sig[i], BorderMode::Clamp);
+    }
+    gaussian_blur_scales(u8, ul, std::vector<double>(sig.begin(), sig.begin() + 3), BorderMode::Clamp);
+    same = true;
+    for (int i = 0; i < 3; i++) same = same && ul[i] == us[i];
+    report("gaussian_blur_scales() on uint8 equals gaussian_blur() per sigma", same);
+}
+
 int main()
 {
     int n = 0;
@@ -364,6 +405,7 @@ int main()
     gradients();
     image_morphology();
     downsample_tests();
+    scale_stack();
     std::printf("%d failures\n", failures);
     return failures;
 }

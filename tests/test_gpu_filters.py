@@ -1,6 +1,10 @@
-"""GPU parity for erode / dilate (processing/morphology.h:57-101) through the
-C ABI (ndl_morph_separable, ndl_morph_nd) against the CPU oracle.  The bar is
-bit-exact for every element type, NaN payloads and signed zeros included."""
+"""GPU parity, through the C ABI, for the filters next to the convolution
+path: erode / dilate (processing/morphology.h:57-101; ndl_morph_separable,
+ndl_morph_nd), downsample (convolution.h:241-305; ndl_downsample) and the
+multi-sigma scale stack with its difference-of-Gaussians layers
+(feature_detection.h:91-120; ndl_gaussian_scales, ndl_dog), against the CPU
+oracle.  The bar is bit-exact for every element type (NaN payloads and signed
+zeros included) except FP32 accumulation, which is held to 1e-6 relative."""
 import itertools
 import sys
 
@@ -170,3 +174,45 @@ def test_downsample_strided_source():
     out = ndl_amd.DeviceArray(want.shape, np.float32)
     ndl_amd.downsample_dev(src, out, 3, 1)
     assert same_bits(out.to_numpy(), want)
+
+
+# ----------------------------------------------------------- scale stacks
+def _dog_sigmas(num_scales=5, sigma0=1.6):
+    # detect_keypoints: sigma_i = sigma0 * 2^(i / (numScales - 3)), feature_detection.h:101-107
+    return [sigma0 * 2.0 ** (i / (num_scales - 3)) for i in range(num_scales)]
+
+
+@pytest.mark.parametrize("shape", [(70, 90), (20, 33, 41)])
+def test_scale_stack_and_dog_exact(shape):
+    rng = np.random.default_rng(55)
+    x = rng.random(shape) * 255.0  # detect_keypoints blurs a double copy of the image
+    sig = _dog_sigmas()
+    say("scale stack", shape)
+    s = ndl_amd.DeviceArray.from_numpy(x)
+    levels = [ndl_amd.DeviceArray(shape, np.float64) for _ in sig]
+    ndl_amd.gaussian_scales_dev(s, levels, sig, "reflect")
+    want = [oracle.gaussian_blur(x, sg, "reflect") for sg in sig]
+    for lv, w in zip(levels, want):
+        assert same_bits(lv.to_numpy(), w)
+    dogs = [ndl_amd.DeviceArray(shape, np.float64) for _ in sig[1:]]
+    ndl_amd.dog_dev(levels, dogs)
+    for i, d in enumerate(dogs):
+        assert same_bits(d.to_numpy(), want[i + 1] - want[i])
+
+
+def test_scale_stack_uint8_and_fp32():
+    rng = np.random.default_rng(56)
+    x = rng.integers(0, 256, size=(64, 80)).astype(np.uint8)
+    sig = _dog_sigmas()[:4]
+    s = ndl_amd.DeviceArray.from_numpy(x)
+    levels = [ndl_amd.DeviceArray(x.shape, np.uint8) for _ in sig]
+    ndl_amd.gaussian_scales_dev(s, levels, sig, "clamp", rounding=ndl_amd.abi.ROUND_NEAREST)
+    for lv, sg in zip(levels, sig):
+        assert np.array_equal(lv.to_numpy(), oracle.gaussian_blur(x, sg, "clamp"))
+    xf = rng.random((40, 50, 60)).astype(np.float32)
+    s = ndl_amd.DeviceArray.from_numpy(xf)
+    levels = [ndl_amd.DeviceArray(xf.shape, np.float32) for _ in sig]
+    ndl_amd.gaussian_scales_dev(s, levels, sig, "reflect", accum=ndl_amd.abi.ACCUM_F32)
+    for lv, sg in zip(levels, sig):
+        want = oracle.gaussian_blur(xf.astype(np.float64), sg, "reflect")
+        assert np.max(np.abs(lv.to_numpy() - want) / np.maximum(np.abs(want), 1e-30)) <= 1e-6
