@@ -379,6 +379,22 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     };
     if (yd < 0 && (yd = free_dim(true)) >= 0) role[yd] = 1;
     if (zd < 0 && (zd = free_dim(false)) >= 0) role[zd] = 2;
+    // row march: an image with no dimension left for z marches down its
+    // convolved y axis with the register ring (one wave per 128-column
+    // strip) instead of giving every 64 x 16 tile its own workgroup and a
+    // single slice step.  Work queues over many images always march; a
+    // single image marches when it is large enough to fill the GPU.
+    bool march = false;
+    if (zd < 0 && yd >= 0 && conv_of_dim[yd] >= 0 && il == 1 && !edge && dt == NDL_F32 &&
+        fused_ring_for(k[conv_of_dim[yd]].count) > 0 && env_int("NDL_ROW_MARCH", 1)) {
+        const int64_t ex = src->extent[xd], ey = src->extent[yd];
+        march = tz_override > 0 || (ey >= 256 && ex * ey >= ((int64_t)1 << 20));
+        if (march) {
+            role[yd] = 2;
+            zd = yd;
+            yd = -1;
+        }
+    }
     memset(&fa, 0, sizeof fa);
     fa.src = src->data;
     fa.dst = dst->data;
@@ -431,7 +447,12 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     if (edge && il != 1) return false;
     cfg.ring = fused_ring_for(fa.kz);
     if (cfg.ring < 0) return false;
-    if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) {
+    if (march) {
+        cfg.tx = 128;
+        cfg.ty = 1;
+        cfg.cpt = 2;
+        if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) return false;
+    } else if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) {
         cfg.cpt = 2;
         cfg.ty = 16;
         if (!fused_supported(cfg, dt, fa.kx, fa.ky, tz_override > 0)) return false;
@@ -1246,6 +1267,7 @@ struct PlanGroup {
     bool int_acc = false;
     FusedConfig cfg{};
     size_t lds = 0;
+    uint32_t grid = 0;           // persistent workgroups (0: the plan default)
     std::vector<FusedArgs> jobs;
     std::vector<uint32_t> tile_start;
     void* d_jobs = nullptr;
@@ -1284,6 +1306,11 @@ struct ndl_plan {
     std::vector<int> fallback;
     uint32_t* d_counter = nullptr;
     uint32_t grid = 1024;
+    // work-queue groups run concurrently on side streams forked from the
+    // caller's stream (default priority) and joined back onto it
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1293,6 +1320,9 @@ struct ndl_plan {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
+        for (hipStream_t q : side) (void)hipStreamDestroy(q);
+        for (hipEvent_t e : join) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
         for (auto& v : variants) v.release();
         if (d_counter) (void)hipFree(d_counter);
     }
@@ -1305,6 +1335,11 @@ int launch_groups(ndl_plan* p, const PlanVariant& v, hipStream_t s)
         hipError_t e = launch_fused(o.fa, o.cfg, o.dt, o.int_acc, s);
         if (e != hipSuccess) return hip_fail(e, "fused launch");
     }
+    // groups are independent (own jobs, outputs and counters): with side
+    // streams they run side by side, each waiting for the caller's prior
+    // work (fork) and the caller waiting for all of them (join)
+    const bool par = v.groups.size() > 1 && !p->side.empty();
+    if (par && hipEventRecord(p->fork, s) != hipSuccess) return hip_fail(hipGetLastError(), "fork event");
     for (size_t i = 0; i < v.groups.size(); i++) {
         const PlanGroup& g = v.groups[i];
         FusedQueue q;
@@ -1313,9 +1348,22 @@ int launch_groups(ndl_plan* p, const PlanVariant& v, hipStream_t s)
         q.njobs = (int)g.jobs.size();
         q.total = g.tile_start.back();
         q.counter = p->d_counter + i;
-        q.grid = std::min<uint32_t>(q.total, p->grid);
-        hipError_t e = launch_fused_queue(q, g.cfg, g.dt, g.int_acc, g.lds, s);
+        q.grid = std::min<uint32_t>(q.total, g.grid ? g.grid : p->grid);
+        hipStream_t qs = s;
+        if (par) {
+            qs = p->side[i % p->side.size()];
+            if (i < p->side.size() && hipStreamWaitEvent(qs, p->fork, 0) != hipSuccess)
+                return hip_fail(hipGetLastError(), "fork wait");
+        }
+        hipError_t e = launch_fused_queue(q, g.cfg, g.dt, g.int_acc, g.lds, qs);
         if (e != hipSuccess) return hip_fail(e, "work-queue launch");
+    }
+    if (par) {
+        const size_t ns = std::min(p->side.size(), v.groups.size());
+        for (size_t j = 0; j < ns; j++) {
+            if (hipEventRecord(p->join[j], p->side[j]) != hipSuccess || hipStreamWaitEvent(s, p->join[j], 0) != hipSuccess)
+                return hip_fail(hipGetLastError(), "join");
+        }
     }
     return NDL_OK;
 }
@@ -1344,7 +1392,9 @@ int build_variant(ndl_plan* p, int tz, PlanVariant& v, bool collect_fallback)
         int dt = p->items[c.item].src.dtype;
         PlanGroup* g = nullptr;
         for (auto& gg : v.groups)
-            if (gg.dt == dt && gg.int_acc == c.int_acc && gg.cfg.ring == c.cfg.ring) g = &gg;
+            if (gg.dt == dt && gg.int_acc == c.int_acc && gg.cfg.ring == c.cfg.ring && gg.cfg.ty == c.cfg.ty &&
+                gg.cfg.tx == c.cfg.tx)
+                g = &gg;
         if (!g) {
             v.groups.emplace_back();
             g = &v.groups.back();
@@ -1352,6 +1402,10 @@ int build_variant(ndl_plan* p, int tz, PlanVariant& v, bool collect_fallback)
             g->int_acc = c.int_acc;
             g->cfg = c.cfg;
             g->tile_start.push_back(0);
+            // as many waves as the 512-thread default: one-wave row-march
+            // workgroups get eight times the workgroups
+            const int nt = c.cfg.tx * c.cfg.ty / c.cfg.cpt;
+            g->grid = (uint32_t)device_cus() * (uint32_t)env_int("NDL_PLAN_WG_PER_CU", 4) * (uint32_t)std::max(1, 512 / nt);
         }
         g->lds = std::max(g->lds, fused_lds_bytes(c.fa, c.cfg, false));
         g->jobs.push_back(c.fa);
@@ -1461,6 +1515,19 @@ NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accu
     p->grid = (uint32_t)std::max(1, cus) * (uint32_t)env_int("NDL_PLAN_WG_PER_CU", 4);
     hipError_t e = hipMalloc((void**)&p->d_counter, sizeof(uint32_t) * max_groups);
     if (e != hipSuccess) return fail(NDL_ERR_NOMEM, "plan counter: %s", hipGetErrorString(e));
+    const int nside = max_groups > 1 ? std::min<int>((int)max_groups, env_int("NDL_PLAN_STREAMS", 4)) : 0;
+    if (nside > 0) {
+        if ((e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming)) != hipSuccess)
+            return hip_fail(e, "plan fork event");
+        for (int i = 0; i < nside; i++) {
+            hipStream_t q = nullptr;
+            hipEvent_t ev = nullptr;
+            if ((e = hipStreamCreateWithFlags(&q, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "plan stream");
+            p->side.push_back(q);
+            if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "plan event");
+            p->join.push_back(ev);
+        }
+    }
     *plan = p.release();
     return NDL_OK;
 }

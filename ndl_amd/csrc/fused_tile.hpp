@@ -186,12 +186,15 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     constexpr int XG = TX / CPT;
     static_assert(CPT >= 1 && XG * CPT == TX && XG * TY == NT, "tile must map CPT x-outputs per thread");
     constexpr int RY = 4;
-    static_assert(TY % RY == 0, "TY must be a multiple of 4");
+    // TY == 1: row march -- one tile row, no y-pass; the x-pass reads the
+    // staged input row and the marched axis (z) is an image's y axis
+    constexpr bool NOY = TY == 1;
+    static_assert(NOY ? (KYS == 1 && !EDGE && IL == 1) : TY % RY == 0, "TY must be 1 or a multiple of 4");
     constexpr int KMAX = 25;
     constexpr int NCMAX = TX + ((KXS > 0 ? KXS : KMAX) - 1) * IL;
     constexpr int NRMAX = TY + (KYS > 0 ? KYS : KMAX) - 1;
     constexpr int NLMAX = (NCMAX * NRMAX + NT - 1) / NT;
-    constexpr int NTASKMAX = (NCMAX * (TY / RY) + NT - 1) / NT;
+    constexpr int NTASKMAX = NOY ? 1 : (NCMAX * (TY / RY) + NT - 1) / NT;
 
     const int kx = KXS > 0 ? KXS : a.kx;
     const int ky = KYS > 0 ? KYS : a.ky;
@@ -388,7 +391,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
         float xv[CPT];
 #pragma unroll
         for (int o = 0; o < CPT; o++) xv[o] = 0.f;
-        const float* row = yp + ry * NCP + xs;
+        const float* row = NOY ? yp + xs : yp + ry * NCP + xs;
         correlate<CPT, KXS, IL>(xv, kx, wxp, [&](int i) NDL_INLINE { return row[i]; });
         const int zout = z0 + s - (P - 1);
         const float* wedge = nullptr;
@@ -451,8 +454,13 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
                 if (s + 3 < nsl) load_slice(s + 3, pre);
             }
         }
-        if (s < nsl) y_pass(even ? sIn0 : sIn1, even ? sY0 : sY1);
-        if (s >= 1) xz_pass(IC<(u + P - 1) % P>{}, s - 1, even ? sY1 : sY0);
+        if constexpr (NOY) {
+            // the x-pass of slice s reads its window straight from sIn
+            if (s < nsl) xz_pass(IC<u % P>{}, s, even ? sIn0 : sIn1);
+        } else {
+            if (s < nsl) y_pass(even ? sIn0 : sIn1, even ? sY0 : sY1);
+            if (s >= 1) xz_pass(IC<(u + P - 1) % P>{}, s - 1, even ? sY1 : sY0);
+        }
         __syncthreads();
     };
     // 2P steps per iteration keep ring slots and buffers compile-time
@@ -502,7 +510,7 @@ __global__ __launch_bounds__(TX * TY / CPT) void fused_queue_kernel(const FusedA
             else hi = mid;
         }
         lo = __builtin_amdgcn_readfirstlane(lo);
-        fused_tile<TS, TX, TY, CPT, 0, 0, P>(jobs[lo], t - tile_start[lo], smem_raw);
+        fused_tile<TS, TX, TY, CPT, 0, TY == 1 ? 1 : 0, P>(jobs[lo], t - tile_start[lo], smem_raw);
         __syncthreads(); // LDS is reused by the next tile
     }
 }

@@ -13,14 +13,27 @@ __global__ void reset_counter_kernel(uint32_t* c) { *c = 0u; }
 
 // ------------------------------------------------------------- dispatch
 constexpr int kTX = 64, kTY = 16;
+// row march (TY = 1): one wave of 2-output lanes per 128-column tile
+constexpr int kRowTX = 128;
+template <int TY> constexpr int tile_x() { return TY == 1 ? kRowTX : kTX; }
 
 // Instantiation table.  Specialised (compile-time taps) filters first, then
 // generic ring periods per storage type.  f(ts, cpt, kx, ky, ring, il, ty)
 // launches.
 template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int kx, int ky, F&& f)
 {
-    if (cfg.tx != kTX) return hipErrorInvalidValue;
     const int cpt = cfg.cpt, kz = cfg.ring, il = cfg.il;
+    if (cfg.ty == 1) {
+        // row march of 2-D float images: run-time x taps, the y taps are the ring
+        if (cfg.tx != kRowTX || cpt != 2 || il != 1 || ky != 1 || dt != NDL_F32) return hipErrorInvalidValue;
+#define NDL_ROW(KZ) \
+    if (kz == KZ) return f(float{}, IC<2>{}, IC<0>{}, IC<1>{}, IC<KZ>{}, IC<1>{}, IC<1>{});
+        NDL_ROW(3) NDL_ROW(5) NDL_ROW(7) NDL_ROW(9) NDL_ROW(11) NDL_ROW(13) NDL_ROW(15) NDL_ROW(17) NDL_ROW(19)
+        NDL_ROW(21) NDL_ROW(23) NDL_ROW(25)
+#undef NDL_ROW
+        return hipErrorInvalidValue;
+    }
+    if (cfg.tx != kTX) return hipErrorInvalidValue;
     if (cfg.ty == 8) {
         // 64 x 8 tiles: twice the tiles per slice, so longer z chunks fill the GPU
         if (dt == NDL_F32 && kx == 11 && ky == 11 && kz == 19 && il == 1) {
@@ -71,9 +84,20 @@ generic:
 // queue kernels: float storage, generic taps
 template <class F> hipError_t with_queue_instance(const FusedConfig& cfg, int dt, F&& f)
 {
-    if (cfg.tx != kTX || cfg.ty != kTY || cfg.cpt != 2 || cfg.il != 1 || dt != NDL_F32) return hipErrorInvalidValue;
+    if (cfg.cpt != 2 || cfg.il != 1 || dt != NDL_F32) return hipErrorInvalidValue;
+    if (cfg.ty == 1) {
+        if (cfg.tx != kRowTX) return hipErrorInvalidValue;
+        switch (cfg.ring) {
+#define NDL_QR(R) case R: return f(IC<R>{}, IC<1>{});
+            NDL_QR(3) NDL_QR(5) NDL_QR(7) NDL_QR(9) NDL_QR(11) NDL_QR(13) NDL_QR(15) NDL_QR(17) NDL_QR(19) NDL_QR(21)
+            NDL_QR(23) NDL_QR(25)
+#undef NDL_QR
+        }
+        return hipErrorInvalidValue;
+    }
+    if (cfg.tx != kTX || cfg.ty != kTY) return hipErrorInvalidValue;
     switch (cfg.ring) {
-#define NDL_Q(R) case R: return f(IC<R>{});
+#define NDL_Q(R) case R: return f(IC<R>{}, IC<kTY>{});
         NDL_Q(1) NDL_Q(3) NDL_Q(5) NDL_Q(7) NDL_Q(9) NDL_Q(11) NDL_Q(13) NDL_Q(15) NDL_Q(17) NDL_Q(19) NDL_Q(21)
         NDL_Q(23) NDL_Q(25)
 #undef NDL_Q
@@ -107,7 +131,7 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
 bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue)
 {
     if (cfg.edge) return !queue && fused_edge_supported(cfg, dt, kx, ky);
-    if (queue) return with_queue_instance(cfg, dt, [](auto) { return hipSuccess; }) == hipSuccess;
+    if (queue) return with_queue_instance(cfg, dt, [](auto, auto) { return hipSuccess; }) == hipSuccess;
     return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto, auto, auto) { return hipSuccess; }) ==
            hipSuccess;
 }
@@ -120,8 +144,8 @@ hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool
     if (cfg.edge) return launch_fused_edge(a, cfg, dt, lds, s);
     return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il, auto ty) {
         using TS = decltype(ts);
-        constexpr int TY = decltype(ty)::value, CPT = decltype(c)::value, NT = kTX * TY / CPT;
-        hipLaunchKernelGGL((fused_tile_kernel<TS, kTX, TY, CPT, decltype(kxs)::value, decltype(kys)::value,
+        constexpr int TY = decltype(ty)::value, CPT = decltype(c)::value, TX = tile_x<TY>(), NT = TX * TY / CPT;
+        hipLaunchKernelGGL((fused_tile_kernel<TS, TX, TY, CPT, decltype(kxs)::value, decltype(kys)::value,
                                               decltype(r)::value, decltype(il)::value>),
                            dim3(a.ntiles), dim3(NT), lds, s, a);
         return hipGetLastError();
@@ -134,9 +158,9 @@ hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int d
     if (cfg.edge) return hipErrorInvalidValue;
     if (!fused_supported(cfg, dt, 0, 0, true)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, q.counter);
-    return with_queue_instance(cfg, dt, [&](auto r) {
-        constexpr int NT = kTX * kTY / 2;
-        hipLaunchKernelGGL((fused_queue_kernel<float, kTX, kTY, 2, decltype(r)::value>), dim3(q.grid), dim3(NT), lds,
+    return with_queue_instance(cfg, dt, [&](auto r, auto ty) {
+        constexpr int TY = decltype(ty)::value, TX = tile_x<TY>(), NT = TX * TY / 2;
+        hipLaunchKernelGGL((fused_queue_kernel<float, TX, TY, 2, decltype(r)::value>), dim3(q.grid), dim3(NT), lds,
                            s, q.jobs, q.tile_start, q.njobs, q.total, q.counter);
         return hipGetLastError();
     });
@@ -148,10 +172,10 @@ int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t l
     int n = 0;
     hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il, auto ty) {
         using TS = decltype(ts);
-        constexpr int TY = decltype(ty)::value, CPT = decltype(c)::value, NT = kTX * TY / CPT;
+        constexpr int TY = decltype(ty)::value, CPT = decltype(c)::value, TX = tile_x<TY>(), NT = TX * TY / CPT;
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &n,
-            reinterpret_cast<const void*>(&fused_tile_kernel<TS, kTX, TY, CPT, decltype(kxs)::value,
+            reinterpret_cast<const void*>(&fused_tile_kernel<TS, TX, TY, CPT, decltype(kxs)::value,
                                                              decltype(kys)::value, decltype(r)::value,
                                                              decltype(il)::value>),
             NT, lds);
