@@ -225,6 +225,10 @@ int ndl_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
 int ndl_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
 int ndl_memset(void* dst, int value, size_t bytes, void* stream);
 int ndl_stream_synchronize(void* stream);
+/* Page-locked host memory (hipHostMalloc) for staging files into device
+ * images (include/ndl/image_io.h); no counterpart in the reference. */
+int ndl_host_alloc(void** ptr, size_t bytes);
+int ndl_host_free(void* ptr);
 int ndl_device_count(int* count);
 
 const char* ndl_last_error(void); /* thread-local; never NULL */

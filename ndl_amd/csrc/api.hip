@@ -1234,6 +1234,21 @@ NDL_API int ndl_memset(void* dst, int value, size_t bytes, void* stream)
     hipError_t e = hipMemsetAsync(dst, value, bytes, (hipStream_t)stream);
     return e == hipSuccess ? NDL_OK : hip_fail(e, "hipMemsetAsync");
 }
+NDL_API int ndl_host_alloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(NDL_ERR_ARG, "null ptr");
+    hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(NDL_ERR_NOMEM, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return NDL_OK;
+}
+NDL_API int ndl_host_free(void* ptr)
+{
+    hipError_t e = hipHostFree(ptr);
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "hipHostFree");
+}
 NDL_API int ndl_stream_synchronize(void* stream)
 {
     hipError_t e = hipStreamSynchronize((hipStream_t)stream);

@@ -490,8 +490,12 @@ __global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedAr
 // workgroups take tiles from one counter until all `total` tiles of all jobs
 // are done.  tile_start[j] is the first global tile of job j (prefix sums,
 // tile_start[njobs] = total).
+#ifndef NDL_QUEUE_WAVES_PER_EU
+#define NDL_QUEUE_WAVES_PER_EU 1
+#endif
 template <class TS, int TX, int TY, int CPT, int P>
-__global__ __launch_bounds__(TX * TY / CPT) void fused_queue_kernel(const FusedArgs* __restrict__ jobs,
+__global__ __launch_bounds__(TX * TY / CPT) __attribute__((amdgpu_waves_per_eu(NDL_QUEUE_WAVES_PER_EU)))
+void fused_queue_kernel(const FusedArgs* __restrict__ jobs,
                                                                     const uint32_t* __restrict__ tile_start,
                                                                     int njobs, uint32_t total, uint32_t* counter)
 {

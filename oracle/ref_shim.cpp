@@ -10,13 +10,16 @@
 // (image/core.h:228-235) and composes per-axis passes the way a caller of the
 // reference would (double intermediates, as convolution.h:151-185 does);
 // ndl::erode / ndl::dilate (processing/morphology.h:57-101) and
-// ndl::downsample (processing/convolution.h:241-305) likewise.
+// ndl::downsample (processing/convolution.h:241-305) likewise; NRRD::save /
+// NRRD::load (imageIO/NRRD/nrrd.h:114-164, 233-366) write and read the files
+// the image_io tests compare against.
 #include <array>
 #include <cmath>
 #include <cstdint>
 #include <type_traits>
 #include <vector>
 
+#include "imageIO/NRRD/nrrd.h"
 #include "processing/convolution.h"
 #include "processing/morphology.h"
 
@@ -263,6 +266,43 @@ __attribute__((visibility("default"))) int ref_downsample(const ndl_tensor* src,
 {
     if (src->dtype != dst->dtype || factor < 2) return NDL_ERR_ARG;
     NDL_REF_DISPATCH(dispatch_dim_down, src, dst, factor, channel_axis)
+}
+
+// NRRD::save<T> of n-dimensional dense data (0 on success)
+#define NDL_REF_NRRD_SAVE(NAME, T)                                                                    \
+    __attribute__((visibility("default"))) int NAME(const char* path, const T* data, int n, const int* size) \
+    {                                                                                                  \
+        return NRRD::save<T>(path, data, n, size) ? 0 : 1;                                            \
+    }
+NDL_REF_NRRD_SAVE(ref_nrrd_save_u8, unsigned char)
+NDL_REF_NRRD_SAVE(ref_nrrd_save_i16, short)
+NDL_REF_NRRD_SAVE(ref_nrrd_save_u16, unsigned short)
+NDL_REF_NRRD_SAVE(ref_nrrd_save_i32, int)
+NDL_REF_NRRD_SAVE(ref_nrrd_save_u32, unsigned int)
+NDL_REF_NRRD_SAVE(ref_nrrd_save_f32, float)
+NDL_REF_NRRD_SAVE(ref_nrrd_save_f64, double)
+#undef NDL_REF_NRRD_SAVE
+
+// NRRD::load<double>: the file's values converted to double, its sizes and
+// dimension; 0 on success, 1 when the reference's loader fails, 2 when the
+// data does not fit `cap`
+__attribute__((visibility("default"))) int ref_nrrd_load_f64(const char* path, double* out, long cap, int* sizes,
+                                                             int* ndim)
+{
+    double* data = nullptr;
+    std::vector<int> size;
+    if (!NRRD::load<double>(path, &data, &size)) return 1;
+    long n = 1;
+    for (int v : size) n *= v;
+    int rc = 0;
+    if (n > cap || size.size() > 10) rc = 2;
+    else {
+        for (long i = 0; i < n; i++) out[i] = data[i];
+        for (size_t d = 0; d < size.size(); d++) sizes[d] = size[d];
+        *ndim = (int)size.size();
+    }
+    delete[] data;
+    return rc;
 }
 
 // mathHelpers.h _kernelSize

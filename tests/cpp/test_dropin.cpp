@@ -9,6 +9,7 @@
 
 #include <ndl/convolution.h>
 #include <ndl/image.h>
+#include <ndl/image_io.h>
 #include <ndl/morphology.h>
 
 using namespace ndl;
@@ -198,6 +199,42 @@ static void per_axis_fp32()
     DeviceImage<float, 3> dx(x), dy(x.extent());
     gaussian_blur<DeviceImage<float, 3>, DeviceImage<float, 3>, 3>(dx, dy, sig, br, Accumulation::FP32);
     report("DeviceImage path equals the staged host path", dy.to_host() == f32);
+}
+
+// NRRD / raw files straight into device images through pinned host memory
+static void file_staging()
+{
+    OwnedImage<float, 3> x({37, 30, 21});
+    std::mt19937 rng(11);
+    std::uniform_real_distribution<float> u(0, 1);
+    for (auto it = x.begin(); it != x.end(); ++it) *it = u(rng);
+    const std::string nrrd = "/tmp/ndl_dropin_volume.nrrd", raw = "/tmp/ndl_dropin_volume.raw";
+    image_io::save(x, nrrd);
+    image_io::save_raw(x, raw);
+    DeviceImage<float, 3> dn = image_io::load_nrrd_device<float, 3>(nrrd);
+    DeviceImage<float, 3> dr = image_io::load_raw_device<float, 3>(raw, x.extent(), 0);
+    report("load_nrrd_device / load_raw_device upload the file's values", dn.to_host() == x && dr.to_host() == x);
+    std::array<double, 3> sig = {1.0, 2.0, 1.5};
+    std::array<BorderMode, 3> br = {BorderMode::Clamp, BorderMode::Reflect, BorderMode::Wrap};
+    OwnedImage<float, 3> host(x.extent());
+    gaussian_blur<OwnedImage<float, 3>, OwnedImage<float, 3>, 3>(x, host, sig, br, Accumulation::FP32);
+    DeviceImage<float, 3> dy(x.extent());
+    gaussian_blur<DeviceImage<float, 3>, DeviceImage<float, 3>, 3>(dn, dy, sig, br, Accumulation::FP32);
+    image_io::save(dy, "/tmp/ndl_dropin_blurred.nrrd");
+    std::array<int, 3> e{};
+    std::vector<float> back = image_io::load_nrrd<float, 3>(e, "/tmp/ndl_dropin_blurred.nrrd");
+    report("blur of a loaded device volume, saved from the device, equals the host path",
+           e == x.extent() && Image<float, 3>(back.data(), e) == host);
+    // a uint8 file converted to float on the way in
+    OwnedImage<unsigned char, 2> g({64, 48});
+    int i = 0;
+    for (auto it = g.begin(); it != g.end(); ++it) *it = (unsigned char)(i++ * 13);
+    image_io::save(g, "/tmp/ndl_dropin_u8.nrrd");
+    DeviceImage<float, 2> dg = image_io::load_nrrd_device<float, 2>("/tmp/ndl_dropin_u8.nrrd");
+    OwnedImage<float, 2> hg = dg.to_host();
+    bool ok = true;
+    for (const auto& c : g.coordinates()) ok = ok && hg.at(c) == (float)g.at(c);
+    report("load_nrrd_device converts a uint8 file to float", ok);
 }
 
 // adjoints: <A x, w> == <x, A^T w>, and tap gradients vs central differences
@@ -406,6 +443,7 @@ int main()
     image_morphology();
     downsample_tests();
     scale_stack();
+    file_staging();
     std::printf("%d failures\n", failures);
     return failures;
 }
