@@ -27,11 +27,13 @@ __global__ __launch_bounds__(256) void flush_read_kernel(const uint4* __restrict
     if (threadIdx.x == 0) sink[blockIdx.x] = red[0];
 }
 
-// plain streaming copy of n16 16-byte words: the measured-copy ceiling
+// streaming copy of n16 16-byte words, one per thread: the measured-copy
+// ceiling (on MI355X this form moves ~6.4 TB/s for 1 GiB, a grid-stride loop
+// over 4096 workgroups ~4.9 TB/s)
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n16)
 {
-    const size_t step = (size_t)gridDim.x * 256;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += step) b[i] = a[i];
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n16) b[i] = a[i];
 }
 
 extern "C" __attribute__((visibility("default"))) int ndl_bench_flush(const void* buf, size_t bytes, void* sink,
@@ -45,7 +47,8 @@ extern "C" __attribute__((visibility("default"))) int ndl_bench_flush(const void
 extern "C" __attribute__((visibility("default"))) int ndl_bench_copy(const void* src, void* dst, size_t bytes,
                                                                      void* stream)
 {
-    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const uint4*)src, (uint4*)dst,
-                       bytes / 16);
+    const size_t n16 = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint4*)src, (uint4*)dst, n16);
     return hipGetLastError() == hipSuccess ? 0 : -6;
 }

@@ -472,8 +472,18 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     }
 }
 
+// Run-time-tap instances hold up to 153 VGPRs (3 waves per SIMD, one
+// 512-thread workgroup per CU); asking for 4 waves per SIMD spills a few
+// registers (8 at ring period 19, 31 at 25) but lets two workgroups share a
+// CU, which measured faster on the 256-instance batched workload (plan 4.12
+// -> 3.48 ms).  Compile-time-tap instances already fit 5-6 waves.
+#ifndef NDL_QUEUE_WAVES_PER_EU
+#define NDL_QUEUE_WAVES_PER_EU 4
+#endif
 template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL, bool EDGE = false>
-__global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedArgs a)
+__global__ __launch_bounds__(TX * TY / CPT)
+__attribute__((amdgpu_waves_per_eu(KXS == 0 || KYS == 0 ? NDL_QUEUE_WAVES_PER_EU : 1)))
+void fused_tile_kernel(const FusedArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // each XCD gets a contiguous range of tiles when remapping (workgroups
@@ -490,9 +500,6 @@ __global__ __launch_bounds__(TX * TY / CPT) void fused_tile_kernel(const FusedAr
 // workgroups take tiles from one counter until all `total` tiles of all jobs
 // are done.  tile_start[j] is the first global tile of job j (prefix sums,
 // tile_start[njobs] = total).
-#ifndef NDL_QUEUE_WAVES_PER_EU
-#define NDL_QUEUE_WAVES_PER_EU 1
-#endif
 template <class TS, int TX, int TY, int CPT, int P>
 __global__ __launch_bounds__(TX * TY / CPT) __attribute__((amdgpu_waves_per_eu(NDL_QUEUE_WAVES_PER_EU)))
 void fused_queue_kernel(const FusedArgs* __restrict__ jobs,
