@@ -56,8 +56,10 @@ template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int 
     NDL_SPEC(NDL_F32, float, 2, 13, 13, 1) NDL_SPEC(NDL_F32, float, 4, 13, 13, 1)
     // video: sigma 1 (7 taps) on x, y and a 5-tap temporal binomial
     NDL_SPEC(NDL_F32, float, 2, 7, 7, 5) NDL_SPEC(NDL_F32, float, 4, 7, 7, 5)
-    // 9-tap kernels on every axis, float and half-precision storage
+    // 9-tap kernels on every axis, float and half-precision storage; the
+    // two-axis passes of their tap gradients (x through the other axes)
     NDL_SPEC(NDL_F32, float, 2, 9, 9, 9) NDL_SPEC(NDL_BF16, bf16_t, 2, 9, 9, 9) NDL_SPEC(NDL_F16, f16_t, 2, 9, 9, 9)
+    NDL_SPEC(NDL_F32, float, 2, 9, 9, 1) NDL_SPEC(NDL_F32, float, 2, 1, 9, 9)
     // uint8 7-tap box on x and y, planar and 3-channel interleaved
     NDL_SPEC(NDL_U8, uint8_t, 2, 7, 7, 1) NDL_SPEC(NDL_U8, uint8_t, 4, 7, 7, 1)
     NDL_SPEC_IL(NDL_U8, uint8_t, 2, 7, 7, 1, 3)

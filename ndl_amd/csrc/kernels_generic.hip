@@ -500,6 +500,107 @@ __global__ __launch_bounds__(256) void tapgrad_partial_kernel(const TapGradArgs 
     }
 }
 
+// Tap gradient with a compile-time tap count K (odd K <= 25, the common
+// filters); same contract and result layout as tapgrad_partial_kernel.
+// SEG: as there, a thread walks a segment of kTapSeg positions along the
+// axis with its U window in registers, lanes on consecutive lines (each
+// window load coalesced across the wave); element offsets inside an item are
+// 32-bit (the host checks the span).  !SEG (the axis is g's fastest
+// dimension): a wave takes 64 consecutive positions of one line, one per
+// lane, and each tap's U load is a contiguous run shifted by the tap (the
+// K loads of a lane hit the same cache lines as its neighbours').  FP64
+// fused multiply-adds, taps ascending per position.
+template <class TG, class TU, int K, bool SEG>
+__global__ __launch_bounds__(256) void tapgrad_exact_kernel(const TapGradArgs a)
+{
+    double acc[K];
+#pragma unroll
+    for (int t = 0; t < K; t++) acc[t] = 0.0;
+    const int32_t u_as = (int32_t)a.u_as, g_as = (int32_t)a.g_as;
+    const int32_t m = (int32_t)a.m, c = a.center;
+    if constexpr (SEG) {
+        constexpr int NW = kTapSeg + K - 1;
+        const uint32_t step = 256u * gridDim.x;
+        for (uint32_t item = blockIdx.x * 256u + threadIdx.x; item < a.items; item += step) {
+            const uint32_t seg = a.ldiv.div(item);
+            uint32_t rem = item - seg * a.lines;
+            int64_t goff = 0, uoff = 0;
+            for (int d = 0; d < a.nl; d++) {
+                uint32_t q = a.fdiv[d].div(rem);
+                int64_t cc = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+                rem = q;
+                goff += cc * a.gst[d];
+                uoff += cc * a.ust[d];
+            }
+            const int32_t j0 = (int32_t)seg * kTapSeg;
+            const TU* ub = (const TU*)a.u + uoff;
+            const TG* gb = (const TG*)a.g + goff;
+            const bool inside = j0 - c >= 0 && j0 + NW - 1 - c < m;
+            const int32_t last = m - 1 + (K - 1 - c);
+            float win[NW];
+#pragma unroll
+            for (int i = 0; i < NW; i++) {
+                int32_t x = j0 + i - c;
+                // positions past last are read only by outputs past the end
+                // of the axis (never accumulated): clamped to stay in bounds
+                if (!inside) x = x > last ? m - 1 : border_map32(a.border, m, x);
+                win[i] = (float)ub[x * u_as];
+            }
+#pragma unroll
+            for (int r = 0; r < kTapSeg; r++) {
+                if (j0 + r < m) {
+                    const double gv = load_f64(gb + (j0 + r) * g_as);
+#pragma unroll
+                    for (int t = 0; t < K; t++) acc[t] = __builtin_fma(gv, (double)win[r + t], acc[t]);
+                }
+            }
+        }
+    } else {
+        // wave items: (line, 64 consecutive positions)
+        const uint32_t nchunk = (uint32_t)((m + 63) >> 6);
+        const uint32_t witems = a.lines * nchunk;
+        const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), nwaves = gridDim.x * 4u;
+        const int32_t lane = threadIdx.x & 63;
+        for (uint32_t it = wave; it < witems; it += nwaves) {
+            const uint32_t line = it / nchunk, chunk = it - line * nchunk;
+            uint32_t rem = line;
+            int64_t goff = 0, uoff = 0;
+            for (int d = 0; d < a.nl; d++) {
+                uint32_t q = a.fdiv[d].div(rem);
+                int64_t cc = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+                rem = q;
+                goff += cc * a.gst[d];
+                uoff += cc * a.ust[d];
+            }
+            const int32_t j = (int32_t)chunk * 64 + lane;
+            if (j < m) {
+                const TU* ub = (const TU*)a.u + uoff;
+                const double gv = load_f64((const TG*)a.g + goff + (int64_t)j * g_as);
+                const bool inside = j - c >= 0 && j + K - 1 - c < m;
+#pragma unroll
+                for (int t = 0; t < K; t++) {
+                    int32_t x = j + t - c;
+                    if (!inside) x = border_map32(a.border, m, x);
+                    acc[t] = __builtin_fma(gv, (double)(float)ub[x * u_as], acc[t]);
+                }
+            }
+        }
+    }
+    __shared__ double red[4][K];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        double v = acc[t];
+        for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+        if (lane == 0) red[wv][t] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < K) {
+        const int t = threadIdx.x;
+        a.partial[(size_t)blockIdx.x * K + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+    }
+}
+
 // one workgroup per tap: fixed-order tree over the workgroup partials
 __global__ __launch_bounds__(256) void tapgrad_final_kernel(const double* partial, int nblocks, int K, double* out)
 {
@@ -519,6 +620,21 @@ __global__ __launch_bounds__(256) void tapgrad_final_kernel(const double* partia
 hipError_t launch_tapgrad_partial(const TapGradArgs& a, int g_dt, int u_is_f64, hipStream_t s)
 {
     if (a.K < 1 || a.K > 64) return hipErrorInvalidValue;
+    // float U (FP32 accumulation), float g: exact-K instances when the
+    // 32-bit item offsets cannot overflow
+    const int64_t span = (a.m + kTapSeg + a.K) * (std::max(std::llabs(a.u_as), std::llabs(a.g_as)) + 1);
+    if (!u_is_f64 && g_dt == NDL_F32 && (a.K & 1) && a.K <= 25 && span < 0x7fffffffLL) {
+        switch (a.K) {
+#define NDL_TGX(K)                                                                                           \
+    case K:                                                                                                  \
+        if (a.seg) hipLaunchKernelGGL((tapgrad_exact_kernel<float, float, K, true>), dim3(a.nblocks), dim3(256), 0, s, a); \
+        else hipLaunchKernelGGL((tapgrad_exact_kernel<float, float, K, false>), dim3(a.nblocks), dim3(256), 0, s, a); \
+        return hipGetLastError();
+            NDL_TGX(1) NDL_TGX(3) NDL_TGX(5) NDL_TGX(7) NDL_TGX(9) NDL_TGX(11) NDL_TGX(13) NDL_TGX(15) NDL_TGX(17)
+            NDL_TGX(19) NDL_TGX(21) NDL_TGX(23) NDL_TGX(25)
+#undef NDL_TGX
+        }
+    }
     auto go = [&](auto tu) {
         using TU = decltype(tu);
         return with_dtype(g_dt, [&](auto tg) {

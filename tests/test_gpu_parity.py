@@ -342,6 +342,28 @@ def test_adjoint_taps_f32(border):
     assert np.all(np.abs(got - want) <= 2e-6 * bound)
 
 
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+def test_adjoint_taps_f32_long_taps_short_axes(border):
+    # compile-time tap counts up to 25 with reach 12 on an axis of 30
+    # (a partly filled 64-position wave item on the fastest axis) and an
+    # axis the segments do not divide
+    rng = np.random.default_rng(23)
+    shape = (50, 26, 30)
+    taps = [rng.standard_normal(k) for k in (25, 7, 11)]
+    x = rng.random(shape).astype(np.float32)
+    w = rng.standard_normal(shape).astype(np.float32)
+    say("adjoint taps f32 long", border)
+    want = oracle.adjoint_taps(x.astype(np.float64), w.astype(np.float64), [0, 1, 2], taps, [border] * 3)
+    bound = oracle.adjoint_taps(np.abs(x.astype(np.float64)), np.abs(w.astype(np.float64)), [0, 1, 2],
+                                [np.abs(t) for t in taps], [border] * 3)
+    dx = ndl_amd.DeviceArray.from_numpy(x)
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    out = ndl_amd.DeviceArray((43,), np.float64)
+    ndl_amd.adjoint_taps_dev(dx, dw, [0, 1, 2], taps, [border] * 3, out, F32)
+    got = out.to_numpy()
+    assert np.all(np.abs(got - want) <= 2e-6 * bound)
+
+
 # ------------------------------------------------------------------ batched
 def test_plan_matches_single_calls():
     rng = np.random.default_rng(20)
