@@ -384,12 +384,29 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     // strip) instead of giving every 64 x 16 tile its own workgroup and a
     // single slice step.  Work queues over many images always march; a
     // single image marches when it is large enough to fill the GPU.
+    // A short untouched dimension in the z role (colour channels) would give
+    // every tile only a few slice steps: it becomes a batch dimension and
+    // the image marches down y instead.
     bool march = false;
-    if (zd < 0 && yd >= 0 && conv_of_dim[yd] >= 0 && il == 1 && !edge && dt == NDL_F32 &&
+    const bool z_short_free = zd >= 0 && conv_of_dim[zd] < 0 && src->extent[zd] <= 4;
+    if ((zd < 0 || z_short_free) && yd >= 0 && conv_of_dim[yd] >= 0 && !edge &&
         fused_ring_for(k[conv_of_dim[yd]].count) > 0 && env_int("NDL_ROW_MARCH", 1)) {
         const int64_t ex = src->extent[xd], ey = src->extent[yd];
         march = tz_override > 0 || (ey >= 256 && ex * ey >= ((int64_t)1 << 20));
         if (march) {
+            // only where a row-march instance exists
+            FusedConfig mc{};
+            mc.tx = 128;
+            mc.ty = 1;
+            mc.cpt = 2;
+            mc.il = il;
+            mc.kmax = 25;
+            mc.ring = fused_ring_for(k[conv_of_dim[yd]].count);
+            const int mkx = conv_of_dim[xd] >= 0 ? k[conv_of_dim[xd]].count : 1;
+            march = fused_supported(mc, dt, mkx, 1, tz_override > 0);
+        }
+        if (march) {
+            if (z_short_free) role[zd] = -1; // a batch dimension now
             role[yd] = 2;
             zd = yd;
             yd = -1;

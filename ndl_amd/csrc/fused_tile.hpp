@@ -189,7 +189,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     // TY == 1: row march -- one tile row, no y-pass; the x-pass reads the
     // staged input row and the marched axis (z) is an image's y axis
     constexpr bool NOY = TY == 1;
-    static_assert(NOY ? (KYS == 1 && !EDGE && IL == 1) : TY % RY == 0, "TY must be 1 or a multiple of 4");
+    static_assert(NOY ? (KYS == 1 && !EDGE) : TY % RY == 0, "TY must be 1 or a multiple of 4");
     constexpr int KMAX = 25;
     constexpr int NCMAX = TX + ((KXS > 0 ? KXS : KMAX) - 1) * IL;
     constexpr int NRMAX = TY + (KYS > 0 ? KYS : KMAX) - 1;

@@ -24,8 +24,14 @@ template <class F> hipError_t with_instance(const FusedConfig& cfg, int dt, int 
 {
     const int cpt = cfg.cpt, kz = cfg.ring, il = cfg.il;
     if (cfg.ty == 1) {
-        // row march of 2-D float images: run-time x taps, the y taps are the ring
-        if (cfg.tx != kRowTX || cpt != 2 || il != 1 || ky != 1 || dt != NDL_F32) return hipErrorInvalidValue;
+        if (cfg.tx != kRowTX || cpt != 2 || ky != 1) return hipErrorInvalidValue;
+        // uint8 7-tap box on x and y (planar and 3-channel interleaved rows)
+        if (dt == NDL_U8 && kx == 7 && kz == 7 && (il == 1 || il == 3)) {
+            if (il == 1) return f(uint8_t{}, IC<2>{}, IC<7>{}, IC<1>{}, IC<7>{}, IC<1>{}, IC<1>{});
+            return f(uint8_t{}, IC<2>{}, IC<7>{}, IC<1>{}, IC<7>{}, IC<3>{}, IC<1>{});
+        }
+        // float images: run-time x taps, the y taps are the ring
+        if (il != 1 || dt != NDL_F32) return hipErrorInvalidValue;
 #define NDL_ROW(KZ) \
     if (kz == KZ) return f(float{}, IC<2>{}, IC<0>{}, IC<1>{}, IC<KZ>{}, IC<1>{}, IC<1>{});
         NDL_ROW(3) NDL_ROW(5) NDL_ROW(7) NDL_ROW(9) NDL_ROW(11) NDL_ROW(13) NDL_ROW(15) NDL_ROW(17) NDL_ROW(19)

@@ -86,6 +86,22 @@ def test_config2a_3d_reflect(n):
 
 
 @pytest.mark.parametrize("interleaved", [False, True])
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+def test_u8_box_row_march_bit_exact(interleaved, border):
+    # colour images large enough to march down y (the channel axis becomes a
+    # batch dimension); widths and heights the 128-column strips and the z
+    # chunks do not divide
+    w, h = 1500, 701
+    v = workloads.uniform_ints(12, 0, w * h * 3, 0, 255).astype(np.uint8)
+    x = v.reshape(h, w, 3) if interleaved else v.reshape(3, h, w)
+    axes = [1, 2] if interleaved else [0, 1]
+    box = [np.ones(7), np.ones(7)]
+    say("u8 row march", interleaved, border)
+    want = oracle.sepconv(x, axes, box, [border] * 2)
+    np.testing.assert_array_equal(gpu_sep(x, axes, box, [border] * 2, REF), want)
+
+
+@pytest.mark.parametrize("interleaved", [False, True])
 def test_config2b_u8_box_wrap_bit_exact(interleaved):
     x, axes, taps, borders = workloads.config2b(interleaved=interleaved)
     want = oracle.sepconv(x, axes, taps, borders)
