@@ -272,9 +272,9 @@ int device_cus()
 int occupancy(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int, int, int, int, int, size_t>, int> cache;
+    static std::map<std::tuple<int, int, int, int, int, int, int, int, int, int, size_t>, int> cache;
     std::lock_guard<std::mutex> lock(mu);
-    auto key = std::make_tuple(dt, cfg.ty, cfg.cpt, cfg.ring, cfg.il, cfg.edge, kx, ky, lds);
+    auto key = std::make_tuple(dt, cfg.tx, cfg.ty, cfg.cpt, cfg.ring, cfg.il, cfg.edge, cfg.grad, kx, ky, lds);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int b = fused_blocks_per_cu(cfg, dt, kx, ky, lds);
@@ -290,8 +290,12 @@ int occupancy(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 // and z roles (-1: none).
 bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                 const ndl_axis_kernel* k, int accum, FusedArgs& fa, FusedConfig& cfg, bool& int_acc,
-                int tz_override = 0, bool edge = false, int* role_dims = nullptr)
+                int tz_override = 0, bool edge = false, int* role_dims = nullptr,
+                const ndl_axis_kernel* grad_k = nullptr, int grad_dim = -1)
 {
+    // grad_k: plan the tap gradient of dimension grad_dim (dst is grad_out):
+    // x goes through the given axes in the tile and is correlated with
+    // grad_out along grad_dim, which must take the z role
     if (env_int("NDL_DISABLE_FUSED", 0)) return false;
     if (src->dtype != dst->dtype || naxes > 3) return false;
     const int dt = src->dtype;
@@ -405,6 +409,7 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
             const int mkx = conv_of_dim[xd] >= 0 ? k[conv_of_dim[xd]].count : 1;
             march = fused_supported(mc, dt, mkx, 1, tz_override > 0);
         }
+        if (march && grad_k) march = false;
         if (march) {
             if (z_short_free) role[zd] = -1; // a batch dimension now
             role[yd] = 2;
@@ -412,6 +417,7 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
             yd = -1;
         }
     }
+    if (grad_k && (zd != grad_dim || conv_of_dim[zd] >= 0 || il != 1 || dt != NDL_F32)) return false;
     memset(&fa, 0, sizeof fa);
     fa.src = src->data;
     fa.dst = dst->data;
@@ -454,6 +460,13 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     set_axis(xd, fa.kx, fa.cx, fa.bx, fa.wx, false);
     set_axis(yd, fa.ky, fa.cy, fa.by, fa.wy, false);
     set_axis(zd, fa.kz, fa.cz, fa.bz, fa.wzlag, true);
+    cfg.grad = grad_k ? 1 : 0;
+    if (grad_k) {
+        // the ring spans the gradient's taps; no z taps are applied
+        fa.kz = grad_k->count;
+        fa.cz = grad_k->center;
+        fa.bz = grad_k->border;
+    }
     cfg.kmax = 25;
     cfg.tx = 64;
     cfg.ty = env_int("NDL_FUSED_TY", 16) == 8 ? 8 : 16;
@@ -464,7 +477,11 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     if (edge && il != 1) return false;
     cfg.ring = fused_ring_for(fa.kz);
     if (cfg.ring < 0) return false;
-    if (march) {
+    if (grad_k) {
+        cfg.ty = 16;
+        cfg.cpt = 2;
+        if (!fused_supported(cfg, dt, fa.kx, fa.ky, false)) return false;
+    } else if (march) {
         cfg.tx = 128;
         cfg.ty = 1;
         cfg.cpt = 2;
@@ -728,6 +745,27 @@ int adjoint_taps_impl(const ndl_tensor* x, const ndl_tensor* g, int32_t naxes, c
         if (oax.empty()) {
             oax.push_back(axes[a]);
             ok.push_back(ndl_axis_kernel{&one, 1, 0, NDL_CLAMP});
+        } else if (accum == NDL_ACCUM_F32 && env_int("NDL_FUSED_TAPGRAD", 1)) {
+            // float32: U_a built in the tile and correlated with grad_out
+            // along axis a without a trip through HBM (axes other than the
+            // unit-stride one); per-tile partial sums, fixed-order final sum
+            FusedArgs fa;
+            FusedConfig cfg;
+            bool int_acc = false;
+            if (plan_fused(x, g, (int)oax.size(), oax.data(), ok.data(), accum, fa, cfg, int_acc, 0, false, nullptr,
+                           &k[a], axes[a])) {
+                Scratch gp;
+                if ((rc = alloc_scratch(gp, (size_t)fa.ntiles * k[a].count * sizeof(double), s))) return rc;
+                fa.gpart = (double*)gp.p;
+                hipError_t e = launch_fused(fa, cfg, x->dtype, int_acc, s);
+                if (e == hipSuccess) {
+                    e = launch_tapgrad_final((const double*)gp.p, (int)fa.ntiles, k[a].count, grad_taps + off, s);
+                    if (e != hipSuccess) return hip_fail(e, "tap-gradient launch");
+                    off += k[a].count;
+                    continue;
+                }
+                if (e != hipErrorInvalidValue) return hip_fail(e, "fused tap-gradient launch");
+            }
         }
         // FP32 accumulation: the fused kernel when it applies (forward_impl
         // falls back to the chain); exact mode: the FP64 chain

@@ -170,6 +170,28 @@ __device__ __forceinline__ void ring_step(float (&ring)[CPT][P], const float (&x
     }
 }
 
+// Tap-gradient step (GRAD): the ring keeps U = x through the x/y taps, and
+// once P slices are in, output z = z0 + s - (P-1) adds g[z] * U[B(z + t - c)]
+// for every tap t = P-1-l (lag l) to FP64 accumulators.  Positions outside
+// the image read g as 0.
+template <int U, int P, int CPT, class TS>
+__device__ __forceinline__ void grad_step(float (&ring)[CPT][P], const float (&xv)[CPT], int s, const FusedArgs& a,
+                                          const TS* g, bool row_ok, int x_first, bool full, double (&gacc)[P])
+{
+#pragma unroll
+    for (int o = 0; o < CPT; o++) ring[o][U] = xv[o];
+    if (s < P - 1) return;
+    float gv[CPT];
+#pragma unroll
+    for (int o = 0; o < CPT; o++) gv[o] = (full || (row_ok && x_first + o < a.X)) ? load_s<TS>(g + o) : 0.f;
+#pragma unroll
+    for (int l = P - 1; l >= 0; l--) {
+#pragma unroll
+        for (int o = 0; o < CPT; o++)
+            gacc[P - 1 - l] = __builtin_fma((double)gv[o], (double)ring[o][(U - l + P) % P], gacc[P - 1 - l]);
+    }
+}
+
 // One tile: `tile` indexes (x tile, y tile, z chunk, batch) with x fastest.
 // KXS/KYS: compile-time x/y tap counts, 0 = taken from the arguments.
 // LDS (floats): sIn = (NR+3) x NC input window (+3 rows of slack for the
@@ -178,9 +200,10 @@ __device__ __forceinline__ void ring_step(float (&ring)[CPT][P], const float (&x
 // EDGE: rows within an axis's edge band (a.eL / a.eR rows at each end, one
 // weight row each in a.etab, window order) take their own weights in place of
 // the band: the transposed clamp / asymmetric-reflect operator.
-template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL = 1, bool EDGE = false>
+template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL = 1, bool EDGE = false, bool GRAD = false>
 __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, unsigned char* smem_raw)
 {
+    static_assert(!GRAD || (!EDGE && TY > 1), "tap gradients use the plain 2-D tile");
     static_assert(!EDGE || IL == 1, "edge rows are for the unit-interleave tile");
     constexpr int NT = TX * TY / CPT;
     constexpr int XG = TX / CPT;
@@ -347,6 +370,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
     for (int o = 0; o < CPT; o++)
 #pragma unroll
         for (int p = 0; p < P; p++) ring[o][p] = 0.f;
+    double gacc[GRAD ? P : 1];
+#pragma unroll
+    for (int t = 0; t < (GRAD ? P : 1); t++) gacc[t] = 0.0;
 
     // y-pass of one slice: window `in` -> y-filtered plane `yp`
     // one y-pass task: column c of the window, output rows [r0, r0 + RY)
@@ -415,7 +441,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
             }
         }
         TS* out = dst + (int64_t)zout * a.dz + (int64_t)(y0 + ry) * a.dy + (x0 + xs);
-        ring_step<U, P, CPT, EDGE, TS>(ring, xv, s, a, out, row_ok, x0 + xs, wedge, full);
+        if constexpr (GRAD) grad_step<U, P, CPT, TS>(ring, xv, s, a, out, row_ok, x0 + xs, full, gacc);
+        else ring_step<U, P, CPT, EDGE, TS>(ring, xv, s, a, out, row_ok, x0 + xs, wedge, full);
     };
     auto publish = [&](const float (&v)[NLMAX], float* in) NDL_INLINE {
 #pragma unroll
@@ -470,6 +497,25 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
             if (s <= nsl) step(ic, s);
         });
     }
+    if constexpr (GRAD) {
+        // the tile's partial sums: waves by shuffles, then the NT/64 wave
+        // values in a fixed order (the march ended with a barrier, so sIn0
+        // is free)
+        double* red = (double*)sIn0;
+        const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int t = 0; t < P; t++) {
+            double v = gacc[t];
+            for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+            if (lane == 0) red[wv * P + t] = v;
+        }
+        __syncthreads();
+        if (tid < P) {
+            double sum = 0.0;
+            for (int w = 0; w < NT / 64; w++) sum = sum + red[w * P + tid];
+            a.gpart[(size_t)tile * P + tid] = sum;
+        }
+    }
 }
 
 // Run-time-tap instances hold up to 153 VGPRs (3 waves per SIMD, one
@@ -480,7 +526,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, uint32_t tile, un
 #ifndef NDL_QUEUE_WAVES_PER_EU
 #define NDL_QUEUE_WAVES_PER_EU 4
 #endif
-template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL, bool EDGE = false>
+template <class TS, int TX, int TY, int CPT, int KXS, int KYS, int P, int IL, bool EDGE = false, bool GRAD = false>
 __global__ __launch_bounds__(TX * TY / CPT)
 __attribute__((amdgpu_waves_per_eu(KXS == 0 || KYS == 0 ? NDL_QUEUE_WAVES_PER_EU : 1)))
 void fused_tile_kernel(const FusedArgs a)
@@ -493,7 +539,7 @@ void fused_tile_kernel(const FusedArgs a)
         const uint32_t per = a.ntiles >> 3;
         tile = (tile & 7u) * per + (tile >> 3);
     }
-    fused_tile<TS, TX, TY, CPT, KXS, KYS, P, IL, EDGE>(a, tile, smem_raw);
+    fused_tile<TS, TX, TY, CPT, KXS, KYS, P, IL, EDGE, GRAD>(a, tile, smem_raw);
 }
 
 // Persistent work queue over many images with the same z tap count:

@@ -138,6 +138,7 @@ size_t fused_lds_bytes(const FusedArgs& a, const FusedConfig& cfg, bool)
 
 bool fused_supported(const FusedConfig& cfg, int dt, int kx, int ky, bool queue)
 {
+    if (cfg.grad) return !queue && fused_grad_supported(cfg, dt, kx, ky);
     if (cfg.edge) return !queue && fused_edge_supported(cfg, dt, kx, ky);
     if (queue) return with_queue_instance(cfg, dt, [](auto, auto) { return hipSuccess; }) == hipSuccess;
     return with_instance(cfg, dt, kx, ky, [](auto, auto, auto, auto, auto, auto, auto) { return hipSuccess; }) ==
@@ -149,6 +150,7 @@ hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool
     if (a.ntiles == 0) return hipSuccess;
     if (cfg.ring != a.kz) return hipErrorInvalidValue;
     const size_t lds = fused_lds_bytes(a, cfg, false);
+    if (cfg.grad) return launch_fused_grad(a, cfg, dt, lds, s);
     if (cfg.edge) return launch_fused_edge(a, cfg, dt, lds, s);
     return with_instance(cfg, dt, a.kx, a.ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il, auto ty) {
         using TS = decltype(ts);
@@ -176,6 +178,7 @@ hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int d
 
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds)
 {
+    if (cfg.grad) return fused_grad_blocks_per_cu(cfg, dt, kx, ky, lds);
     if (cfg.edge) return fused_edge_blocks_per_cu(cfg, dt, kx, ky, lds);
     int n = 0;
     hipError_t e = with_instance(cfg, dt, kx, ky, [&](auto ts, auto c, auto kxs, auto kys, auto r, auto il, auto ty) {

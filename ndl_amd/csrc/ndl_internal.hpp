@@ -178,12 +178,17 @@ struct FusedArgs {
     // everywhere.  Only the EDGE instances read them.
     const float* etab[3];
     int32_t eL[3], eR[3];
+    // tap-gradient instances (FusedConfig::grad): dst is grad_out (read),
+    // the ring holds x passed through the x/y taps, and each tile writes the
+    // kz partial sums sum_p g[p] * U[B_z(p_z + t - cz)] to gpart[tile * kz + t]
+    double* gpart;
 };
 struct FusedConfig {
     int tx, ty, ring, kmax;       // template selection
     int cpt;                      // x outputs per thread (threads = tx*ty/cpt)
     int il;                       // interleave: x taps il elements apart (channels)
     int edge;                     // 1: rows near the ends take weight rows (FusedArgs::etab)
+    int grad;                     // 1: tap gradient of the z role (FusedArgs::gpart)
 };
 // Returns hipErrorInvalidValue when no instantiation matches cfg/dtype.
 hipError_t launch_fused(const FusedArgs& a, const FusedConfig& cfg, int dt, bool int_acc,
@@ -204,6 +209,10 @@ int fused_ring_for(int kz);       // ring period for kz, or -1 when unsupported
 hipError_t launch_fused_edge(const FusedArgs& a, const FusedConfig& cfg, int dt, size_t lds, hipStream_t s);
 bool fused_edge_supported(const FusedConfig& cfg, int dt, int kx, int ky);
 int fused_edge_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
+// Tap-gradient instances (cfg.grad), kernels_fused_grad.hip.
+hipError_t launch_fused_grad(const FusedArgs& a, const FusedConfig& cfg, int dt, size_t lds, hipStream_t s);
+bool fused_grad_supported(const FusedConfig& cfg, int dt, int kx, int ky);
+int fused_grad_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
 
 // Edge weight rows of transposed axis operators, built on the device:
 // for axis a of extent M with taps w (count K, centre c) and border B,

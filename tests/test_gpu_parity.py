@@ -336,8 +336,12 @@ def test_adjoint_taps(border):
     np.testing.assert_allclose(out.to_numpy(), want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
 
 
+@pytest.mark.parametrize("fused", ["1", "0"])
 @pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
-def test_adjoint_taps_f32(border):
+def test_adjoint_taps_f32(border, fused, monkeypatch):
+    # fused "1": axes other than the unit-stride one correlate grad_out with
+    # U inside the tile kernel; "0": U materialised, then the reduction kernel
+    monkeypatch.setenv("NDL_FUSED_TAPGRAD", fused)
     # FP32 path: the partial results U come from the fused kernel, the
     # reduction runs in FP64; error bound relative to sum |g * U| (the same
     # reduction over |x|, |g| and |taps|)
@@ -358,8 +362,10 @@ def test_adjoint_taps_f32(border):
     assert np.all(np.abs(got - want) <= 2e-6 * bound)
 
 
+@pytest.mark.parametrize("fused", ["1", "0"])
 @pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
-def test_adjoint_taps_f32_long_taps_short_axes(border):
+def test_adjoint_taps_f32_long_taps_short_axes(border, fused, monkeypatch):
+    monkeypatch.setenv("NDL_FUSED_TAPGRAD", fused)
     # compile-time tap counts up to 25 with reach 12 on an axis of 30
     # (a partly filled 64-position wave item on the fastest axis) and an
     # axis the segments do not divide
