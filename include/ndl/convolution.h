@@ -53,6 +53,18 @@ inline int32_t border_code(BorderMode b)
     return b == BorderMode::Wrap ? NDL_WRAP : (b == BorderMode::Reflect ? NDL_REFLECT : NDL_CLAMP);
 }
 
+template <class T, int DIM> ndl_tensor host_tensor(const std::array<int, DIM>& e, const std::array<int, DIM>& st)
+{
+    ndl_tensor t{};
+    t.dtype = dtype_of<T>();
+    t.ndim = DIM;
+    for (int i = 0; i < DIM; i++) {
+        t.extent[i] = e[i];
+        t.stride[i] = st[i];
+    }
+    return t;
+}
+
 // A source on the device: DeviceImage as is, host views staged by span.
 template <class ImageT> struct DeviceSource {
     using T = typename ImageT::value_type;
@@ -64,8 +76,12 @@ template <class ImageT> struct DeviceSource {
         if constexpr (is_device<ImageT>::value) {
             t = img.tensor();
         } else {
+            static_assert(has_stride_access_v<ImageT, DIM>,
+                          "the GPU path needs an image with linear strides (stride() and a T& at()): Image, "
+                          "OwnedImage, their views, DeviceImage, or a type of your own with that interface; convert "
+                          "bit-packed or ring-buffer images first");
             static_assert(gpu_type_v<T>, "GPU convolution supports u8/i8/u16/i16/i32/float/double images");
-            t = img.tensor(dtype_of<T>());
+            t = host_tensor<T, DIM>(img.extent(), img.stride());
             std::ptrdiff_t lo = 0, hi = 0;
             bool empty = false;
             for (int d = 0; d < DIM; d++) {
@@ -74,9 +90,11 @@ template <class ImageT> struct DeviceSource {
                 if (r < 0) lo += r; else hi += r;
             }
             if (empty) return;
+            // the element at index 0 of every dimension, through the type's own at()
+            const T* origin = &const_cast<ImageT&>(img).at(std::array<int, DIM>{});
             std::size_t bytes = (std::size_t)(hi - lo + 1) * sizeof(T);
             keep.emplace_back(bytes);
-            check(ndl_memcpy_htod(keep.back().p, img.data() + lo, bytes, nullptr));
+            check(ndl_memcpy_htod(keep.back().p, origin + lo, bytes, nullptr));
             t.data = (T*)keep.back().p - lo;
         }
     }
@@ -94,6 +112,9 @@ template <class ImageT> struct DeviceDest {
         if constexpr (is_device<ImageT>::value) {
             t = img.tensor();
         } else {
+            static_assert(has_stride_access_v<ImageT, DIM>,
+                          "the GPU path needs a destination with linear strides (stride() and a T& at()): Image, "
+                          "OwnedImage, their views, DeviceImage, or a type of your own with that interface");
             static_assert(gpu_type_v<T>, "GPU convolution supports u8/i8/u16/i16/i32/float/double images");
             host = &img;
             std::array<int, DIM> e = img.extent();
@@ -110,14 +131,25 @@ template <class ImageT> struct DeviceDest {
             check(ndl_stream_synchronize(nullptr));
             std::size_t n = Image<T, DIM>::size(host->extent());
             if (n == 0) return;
-            if (host->stride() == default_stride<DIM>(host->extent())) {
-                check(ndl_memcpy_dtoh(host->data(), t.data, n * sizeof(T), nullptr));
-                check(ndl_stream_synchronize(nullptr));
-            } else {
-                std::vector<T> tmp(n);
-                check(ndl_memcpy_dtoh(tmp.data(), t.data, n * sizeof(T), nullptr));
-                check(ndl_stream_synchronize(nullptr));
-                host->assign(Image<T, DIM>(tmp.data(), host->extent()));
+            if constexpr (std::is_base_of_v<Image<T, DIM>, ImageT>) {
+                if (host->stride() == default_stride<DIM>(host->extent())) {
+                    check(ndl_memcpy_dtoh(host->data(), t.data, n * sizeof(T), nullptr));
+                    check(ndl_stream_synchronize(nullptr));
+                    return;
+                }
+            }
+            std::vector<T> tmp(n);
+            check(ndl_memcpy_dtoh(tmp.data(), t.data, n * sizeof(T), nullptr));
+            check(ndl_stream_synchronize(nullptr));
+            // through the destination's own at(): elements outside a view stay untouched
+            Image<T, DIM> dense(tmp.data(), host->extent());
+            std::array<int, DIM> c{};
+            for (std::size_t i = 0; i < n; i++) {
+                host->at(c) = dense.at(c);
+                for (int d = 0; d < DIM; d++) {
+                    if (++c[d] < host->extent()[d]) break;
+                    c[d] = 0;
+                }
             }
         } else {
             check(ndl_stream_synchronize(nullptr));

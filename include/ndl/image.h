@@ -348,6 +348,40 @@ struct DeviceBuffer {
 } // namespace detail
 
 /// A dense image in device memory (x fastest).  Owning; move-only.
+// Image-type contracts of the reference's free functions (image/detail.h:
+// 48-57 "minimal interface", 90-98 "stride access"), stated here with a
+// detection helper of this library: a minimal-interface image has extent()
+// (convertible to std::array<int,DIM>), at(coord) and coordinates(); one with
+// stride access also has stride() and an at() returning a real T& into
+// linear memory.  The GPU path accepts every type with stride access (and
+// DeviceImage); bit-packed or ring-buffer images must be converted first.
+namespace detail {
+template <class Void, template <class...> class Op, class... A> struct detected : std::false_type {};
+template <template <class...> class Op, class... A>
+struct detected<std::void_t<Op<A...>>, Op, A...> : std::true_type {};
+template <class I> using extent_expr = decltype(std::declval<const I&>().extent());
+template <class I> using stride_expr = decltype(std::declval<const I&>().stride());
+template <class I> using coordinates_expr = decltype(std::declval<const I&>().coordinates());
+template <class I, class C> using at_expr = decltype(std::declval<I&>().at(std::declval<C>()));
+template <class I, int DIM, bool = detected<void, extent_expr, I>::value> struct extent_is : std::false_type {};
+template <class I, int DIM>
+struct extent_is<I, DIM, true> : std::is_convertible<extent_expr<I>, std::array<int, DIM>> {};
+template <class I, int DIM, bool = detected<void, stride_expr, I>::value> struct stride_is : std::false_type {};
+template <class I, int DIM>
+struct stride_is<I, DIM, true> : std::is_convertible<stride_expr<I>, std::array<int, DIM>> {};
+template <class I, int DIM, bool = detected<void, at_expr, I, std::array<int, DIM>>::value>
+struct at_is_ref : std::false_type {};
+template <class I, int DIM>
+struct at_is_ref<I, DIM, true> : std::is_lvalue_reference<at_expr<I, std::array<int, DIM>>> {};
+
+template <class ImageT, int DIM>
+inline constexpr bool satisfies_minimal_interface_v =
+    extent_is<ImageT, DIM>::value && detected<void, at_expr, const ImageT, std::array<int, DIM>>::value &&
+    detected<void, coordinates_expr, ImageT>::value;
+template <class ImageT, int DIM>
+inline constexpr bool has_stride_access_v = stride_is<ImageT, DIM>::value && at_is_ref<ImageT, DIM>::value;
+} // namespace detail
+
 template <class T, int DIM> class DeviceImage {
     static_assert(detail::gpu_type_v<T>, "DeviceImage<T,DIM>: T must be u8/i8/u16/i16/i32/float/double");
 

@@ -55,6 +55,8 @@ SIGNATURES = {
     "ndl_memcpy_dtoh": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
     "ndl_memset": (C.c_int, [_VP, C.c_int, C.c_size_t, _VP]),
     "ndl_stream_synchronize": (C.c_int, [_VP]),
+    "ndl_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "ndl_host_free": (C.c_int, [_VP]),
     "ndl_device_count": (C.c_int, [_P(C.c_int)]),
     "ndl_last_error": (C.c_char_p, []),
     "ndl_version": (C.c_int, []),

@@ -201,6 +201,44 @@ static void per_axis_fp32()
     report("DeviceImage path equals the staged host path", dy.to_host() == f32);
 }
 
+// an image type of the caller's own (not this library's Image) with the
+// reference's minimal interface plus strides (image/detail.h:48-57,90-98):
+// one channel of an interleaved RGB buffer
+template <class T> struct ChannelView {
+    typedef T value_type;
+    T* base;
+    int w, h, ch;
+    std::array<int, 2> extent() const { return {w, h}; }
+    std::array<int, 2> stride() const { return {3, 3 * w}; }
+    T& at(const std::array<int, 2>& i) const { return base[ch + 3 * (i[0] + w * i[1])]; }
+    std::vector<std::array<int, 2>> coordinates() const
+    {
+        std::vector<std::array<int, 2>> c;
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) c.push_back({x, y});
+        return c;
+    }
+};
+
+static void foreign_image_type()
+{
+    const int w = 53, h = 41;
+    std::vector<float> rgb(3 * w * h), out(3 * w * h, -7.f);
+    std::mt19937 rng(13);
+    std::uniform_real_distribution<float> u(0, 1);
+    for (float& v : rgb) v = u(rng);
+    ChannelView<float> src{rgb.data(), w, h, 1}, dst{out.data(), w, h, 1};
+    gaussian_blur(src, dst, 1.5, BorderMode::Reflect);
+    // the same channel through this library's strided Image view
+    Image<float, 3> all(rgb.data(), {3, w, h});
+    OwnedImage<float, 2> want({w, h});
+    gaussian_blur(all.slice(0, 1), want, 1.5, BorderMode::Reflect);
+    bool ok = true;
+    for (const auto& c : want.coordinates()) ok = ok && dst.at(c) == want.at(c);
+    for (int i = 0; i < w * h; i++) ok = ok && out[3 * i] == -7.f && out[3 * i + 2] == -7.f;
+    report("a caller's own strided image type (minimal interface + stride access) works unchanged", ok);
+}
+
 // NRRD / raw files straight into device images through pinned host memory
 static void file_staging()
 {
@@ -444,6 +482,7 @@ int main()
     downsample_tests();
     scale_stack();
     file_staging();
+    foreign_image_type();
     std::printf("%d failures\n", failures);
     return failures;
 }
