@@ -226,9 +226,104 @@ __global__ __launch_bounds__(256) void morph_pass_kernel(const MorphPassArgs a)
     morph_store<TIn, TOut>((TOut*)a.out + ooff, best);
 }
 
+// The same pass with a compile-time tap count K: NO outputs per thread
+// (thread t takes outputs t, t + T, ..., T = all threads), every window value
+// of all NO outputs loaded before the first comparison, border maps only for
+// windows that reach past an end of the axis.  Comparison order, ties, NaN
+// and the corner rule are those of morph_pass_kernel.
+template <class TIn, class TOut, bool MAX, int K, int NO>
+__global__ __launch_bounds__(256) void morph_pass_k_kernel(const MorphPassArgs a)
+{
+    using Key = typename MorphKey<TIn>::type;
+    const uint32_t nthreads = gridDim.x * 256u;
+    const uint32_t first = blockIdx.x * 256u + threadIdx.x;
+    TIn v[NO][K];
+    int64_t ooff[NO], coff[NO];
+    bool live[NO];
+#pragma unroll
+    for (int o = 0; o < NO; o++) {
+        const uint32_t idx = first + (uint32_t)o * nthreads;
+        live[o] = idx < a.total;
+        ooff[o] = 0;
+        coff[o] = 0;
+        if (!live[o]) continue;
+        int64_t ioff = 0, j = 0;
+        uint32_t rem = idx;
+        for (int d = 0; d < a.nd; d++) {
+            uint32_t q = a.fdiv[d].div(rem);
+            int64_t c = (int64_t)(rem - q * (uint32_t)a.ext[d]);
+            rem = q;
+            ioff += c * a.ist[d];
+            ooff[o] += c * a.ost[d];
+            if (d == a.axis_pos) j = c;
+            if (a.corner) {
+                int64_t x = c + a.coff[d];
+                if (x < 0 || x >= a.ext[d]) x = border_map(a.cbord[d], a.ext[d], x);
+                coff[o] += x * a.cst[d];
+            }
+        }
+        const TIn* in = (const TIn*)a.in + (ioff - j * a.in_as);
+        const bool inside = j - a.center >= 0 && j + K - 1 - a.center < a.m;
+#pragma unroll
+        for (int t = 0; t < K; t++) {
+            int64_t x = j + t - a.center;
+            if (!inside) x = border_map(a.border, a.m, x);
+            v[o][t] = in[x * a.in_as];
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < NO; o++) {
+        if (!live[o]) continue;
+        TIn best = v[o][0];
+        Key bk = morph_key(best);
+#pragma unroll
+        for (int t = 1; t < K; t++) {
+            const Key k = morph_key(v[o][t]);
+            // strict comparison, first tap wins ties; NaN never wins over a number
+            const bool take = key_nan(bk) ? !key_nan(k) : (MAX ? k > bk : k < bk);
+            if (take) {
+                best = v[o][t];
+                bk = k;
+            }
+        }
+        if constexpr (std::is_floating_point<Key>::value) {
+            if (a.corner) {
+                const TIn cv = ((const TIn*)a.corner)[coff[o]];
+                if (key_nan(morph_key(cv))) best = cv;
+            }
+        }
+        morph_store<TIn, TOut>((TOut*)a.out + ooff[o], best);
+    }
+}
+
 hipError_t launch_morph_pass(const MorphPassArgs& a, int in_dt, int out_dt, hipStream_t s)
 {
     if (a.total == 0) return hipSuccess;
+    // every tap included (box elements): compile-time tap counts
+    bool all = a.K >= 1;
+    for (int t = 0; t < a.K; t++) all = all && a.mask[t];
+    // (same-type passes and the common box sizes; the rest take the
+    // run-time-tap kernel below)
+    if (all && in_dt == out_dt && (a.K == 3 || a.K == 5 || a.K == 7)) {
+        constexpr int NO = 4;
+        const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, ((uint64_t)a.total + 256 * NO - 1) / (256 * NO));
+        return with_dtype(in_dt, [&](auto tin) {
+            using TIn = decltype(tin);
+            {
+                using TOut = TIn;
+                switch (a.K) {
+#define NDL_MK(K)                                                                                                   \
+    case K:                                                                                                         \
+        if (a.dilate) hipLaunchKernelGGL((morph_pass_k_kernel<TIn, TOut, true, K, NO>), dim3(blocks), dim3(256), 0, s, a); \
+        else hipLaunchKernelGGL((morph_pass_k_kernel<TIn, TOut, false, K, NO>), dim3(blocks), dim3(256), 0, s, a); \
+        return hipGetLastError();
+                    NDL_MK(3) NDL_MK(5) NDL_MK(7)
+#undef NDL_MK
+                }
+                return hipErrorInvalidValue;
+            }
+        });
+    }
     uint32_t blocks = (a.total + 255u) / 256u;
     return with_dtype(in_dt, [&](auto tin) {
         using TIn = decltype(tin);
