@@ -595,6 +595,17 @@ __global__ __launch_bounds__(256) void tapgrad_partial_kernel(const TapGradArgs 
     }
 }
 
+// Orders one wave's LDS writes before its later reads by other lanes (and
+// those reads before the next writes): the fences keep the compiler from
+// moving LDS accesses across; the hardware runs one wave's LDS instructions
+// in order.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Tap gradient with a compile-time tap count K (odd K <= 25, the common
 // filters); same contract and result layout as tapgrad_partial_kernel.
 // SEG: as there, a thread walks a segment of kTapSeg positions along the
@@ -651,11 +662,17 @@ __global__ __launch_bounds__(256) void tapgrad_exact_kernel(const TapGradArgs a)
             }
         }
     } else {
-        // wave items: (line, 64 consecutive positions)
+        // wave items: (line, 64 consecutive positions).  The wave stages the
+        // item's U window (64 + K - 1 positions, border-mapped) in its own
+        // LDS row with one coalesced load per position; each lane then reads
+        // its K window values from LDS (wave_lds_sync between the two).
+        __shared__ float wrow[4][64 + 32];
         const uint32_t nchunk = (uint32_t)((m + 63) >> 6);
         const uint32_t witems = a.lines * nchunk;
         const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), nwaves = gridDim.x * 4u;
         const int32_t lane = threadIdx.x & 63;
+        float* row = wrow[threadIdx.x >> 6];
+        const int32_t last = m - 1 + (K - 1 - c);
         for (uint32_t it = wave; it < witems; it += nwaves) {
             const uint32_t line = it / nchunk, chunk = it - line * nchunk;
             uint32_t rem = line;
@@ -667,17 +684,26 @@ __global__ __launch_bounds__(256) void tapgrad_exact_kernel(const TapGradArgs a)
                 goff += cc * a.gst[d];
                 uoff += cc * a.ust[d];
             }
-            const int32_t j = (int32_t)chunk * 64 + lane;
-            if (j < m) {
-                const TU* ub = (const TU*)a.u + uoff;
-                const double gv = load_f64((const TG*)a.g + goff + (int64_t)j * g_as);
-                const bool inside = j - c >= 0 && j + K - 1 - c < m;
+            const int32_t j0 = (int32_t)chunk * 64;
+            const TU* ub = (const TU*)a.u + uoff;
+            wave_lds_sync();
 #pragma unroll
-                for (int t = 0; t < K; t++) {
-                    int32_t x = j + t - c;
-                    if (!inside) x = border_map32(a.border, m, x);
-                    acc[t] = __builtin_fma(gv, (double)(float)ub[x * u_as], acc[t]);
+            for (int i0 = 0; i0 < 64 + K - 1; i0 += 64) {
+                const int32_t i = i0 + lane;
+                if (i < 64 + K - 1) {
+                    int32_t x = j0 + i - c;
+                    // positions past last feed only outputs past the end of
+                    // the axis (never accumulated): clamped to stay in bounds
+                    if (x < 0 || x >= m) x = x > last ? m - 1 : border_map32(a.border, m, x);
+                    row[i] = (float)ub[x * u_as];
                 }
+            }
+            wave_lds_sync();
+            const int32_t j = j0 + lane;
+            if (j < m) {
+                const double gv = load_f64((const TG*)a.g + goff + (int64_t)j * g_as);
+#pragma unroll
+                for (int t = 0; t < K; t++) acc[t] = __builtin_fma(gv, (double)row[lane + t], acc[t]);
             }
         }
     }
