@@ -3,7 +3,8 @@
 // TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the
 // reference headers where they lie (-I <reference root>) into
 // oracle/_ref/libndl_ref.so; it is built only when the reference sources are
-// present.  Used to pin the oracle restatement (tests/test_oracle_vs_ref.py)
+// present.  Used to pin the oracle restatement (tests/test_oracle.py,
+// test_oracle_morph.py, test_oracle_downsample.py, test_image_io.py)
 // and, optionally, as the timed CPU baseline.  No reference source is copied:
 // this file only instantiates ndl::convolve / ndl::gaussian_blur
 // (processing/convolution.h:52-71,107-186) on dense Image<T,DIM> views
