@@ -364,6 +364,30 @@ def test_adjoint_taps_f32(border, fused, monkeypatch):
 
 @pytest.mark.parametrize("fused", ["1", "0"])
 @pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+def test_adjoint_taps_f32_nine_taps_every_axis(border, fused, monkeypatch):
+    # config-5 filter shape (9 random taps per axis, the compile-time 9-tap
+    # instances): with fused "1" the y and z gradients run in the tile
+    # kernel; x extent 70 gives one full and one partial 64-column tile
+    monkeypatch.setenv("NDL_FUSED_TAPGRAD", fused)
+    rng = np.random.default_rng(24)
+    shape = (36, 50, 70)
+    taps = [rng.standard_normal(9) for _ in range(3)]
+    x = rng.random(shape).astype(np.float32)
+    w = rng.standard_normal(shape).astype(np.float32)
+    say("adjoint taps f32 9/9/9", border, fused)
+    want = oracle.adjoint_taps(x.astype(np.float64), w.astype(np.float64), [0, 1, 2], taps, [border] * 3)
+    bound = oracle.adjoint_taps(np.abs(x.astype(np.float64)), np.abs(w.astype(np.float64)), [0, 1, 2],
+                                [np.abs(t) for t in taps], [border] * 3)
+    dx = ndl_amd.DeviceArray.from_numpy(x)
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    out = ndl_amd.DeviceArray((27,), np.float64)
+    ndl_amd.adjoint_taps_dev(dx, dw, [0, 1, 2], taps, [border] * 3, out, F32)
+    got = out.to_numpy()
+    assert np.all(np.abs(got - want) <= 2e-6 * bound)
+
+
+@pytest.mark.parametrize("fused", ["1", "0"])
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
 def test_adjoint_taps_f32_long_taps_short_axes(border, fused, monkeypatch):
     monkeypatch.setenv("NDL_FUSED_TAPGRAD", fused)
     # compile-time tap counts up to 25 with reach 12 on an axis of 30
