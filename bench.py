@@ -298,6 +298,21 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
             emit({"config": f"2b: u8 2048x2048x3 box7 wrap ({'interleaved' if inter else 'planar'}), forward",
                   "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
                   "GBps_alg": round(2 * x.size / ms / 1e6, 1), "bit_exact": bool(ok)})
+        # the 256^3 Gaussian's axes one at a time (convolve() with a 1-D
+        # kernel image per axis), cold cache: x on the tile kernel, y and z
+        # on the line march
+        x, axes, taps, borders = workloads.config2a()
+        s = ndl_amd.DeviceArray.from_numpy(x)
+        d = ndl_amd.DeviceArray(x.shape, np.float32)
+        for a, name in zip(axes, "xyz"):
+            f = fwd_fn(s, d, [a], [taps[a]], [borders[a]], abi.ACCUM_F32)
+            f()
+            ms = _time_launches(hip, stream, f, reps, flush)
+            err = float(np.max(np.abs(d.to_numpy().astype(np.float64) -
+                                      oracle.sepconv(x, [a], [taps[a]], [borders[a]]).astype(np.float64))))
+            emit({"config": f"2a one axis: 256^3 f32, {len(taps[a])} taps on {name} only, reflect, cold",
+                  "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
+                  "GBps_alg": round(8 * x.size / ms / 1e6, 1), "max_abs_err": err})
     if 3 in configs:
         x, axes, taps, borders = workloads.config3()
         s = ndl_amd.DeviceArray.from_numpy(x)

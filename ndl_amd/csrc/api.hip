@@ -528,9 +528,91 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     return true;
 }
 
+// One float32 axis that is not the unit-stride one, FP32 accumulation:
+// the line march (kernels_line.hip).  v = x elements per lane, d = loads in
+// flight per lane.
+bool plan_line(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+               const ndl_axis_kernel* k, int accum, LineArgs& la, int& v, int& d)
+{
+    if (naxes != 1 || accum != NDL_ACCUM_F32 || !env_int("NDL_LINE_MARCH", 1)) return false;
+    const int dt = src->dtype;
+    if (dt != NDL_F32 || dst->dtype != NDL_F32) return false;
+    const ndl_axis_kernel& ak = k[0];
+    // a zero tap must not multiply (detail.h:164-174: 0*inf would be NaN)
+    for (int t = 0; t < ak.count; t++)
+        if (ak.taps[t] == 0.0) return false;
+    const int nd = src->ndim, zd = axes[0];
+    int xd = -1;
+    for (int e = 0; e < nd; e++)
+        if (src->stride[e] == 1 && dst->stride[e] == 1 && src->extent[e] > 1) { xd = e; break; }
+    if (xd < 0 || xd == zd || src->extent[xd] < 64 || src->extent[zd] > 0x7fffffff) return false;
+    // widest vector whose every lane address is aligned: x extent, base
+    // addresses and every other stride multiples of v elements
+    auto fits = [&](int vv) {
+        if (src->extent[xd] % vv || (uintptr_t)src->data % (4 * vv) || (uintptr_t)dst->data % (4 * vv)) return false;
+        for (int e = 0; e < nd; e++)
+            if (e != xd && src->extent[e] > 1 && (src->stride[e] % vv || dst->stride[e] % vv)) return false;
+        return true;
+    };
+    v = env_int("NDL_LINE_V", 4);
+    if (v != 1 && v != 2 && v != 4) v = 4;
+    while (v > 1 && !fits(v)) v /= 2;
+    d = env_int("NDL_LINE_D", 8);
+    if (!line_march_supported(dt, ak.count, v, d)) return false;
+    memset(&la, 0, sizeof la);
+    la.src = src->data;
+    la.dst = dst->data;
+    la.Z = (int32_t)src->extent[zd];
+    la.sz = src->stride[zd];
+    la.dz = dst->stride[zd];
+    la.k = ak.count;
+    la.c = ak.center;
+    la.border = ak.border;
+    for (int t = 0; t < ak.count; t++) la.w[t] = (float)ak.taps[t];
+    const int64_t xv = src->extent[xd] / v;
+    int64_t lanes = xv;
+    la.xv.init((uint32_t)xv);
+    for (int e = 0; e < nd; e++) {
+        if (e == xd || e == zd || src->extent[e] == 1) continue;
+        la.bext[la.nb].init((uint32_t)std::min<int64_t>(src->extent[e], 0xffffffffLL));
+        la.sb[la.nb] = src->stride[e];
+        la.db[la.nb] = dst->stride[e];
+        lanes *= src->extent[e];
+        la.nb++;
+        if (lanes > 0xfffffe00LL) return false;
+    }
+    la.lanes = (uint32_t)lanes;
+    // z chunks: fill the resident workgroup slots in whole rounds; a chunk
+    // of tz outputs reads tz + k - 1 slices
+    const int64_t lblocks = (lanes + 255) / 256;
+    const int64_t slots = (int64_t)device_cus() * std::max(1, line_march_blocks_per_cu(dt, ak.count, v, d));
+    int tz = la.Z;
+    double best = 1e300;
+    for (int chunks = 1; chunks <= std::min(la.Z, 256); chunks++) {
+        const int t = (la.Z + chunks - 1) / chunks;
+        const int64_t blocks = lblocks * ((la.Z + t - 1) / t);
+        const double cost = (double)((blocks + slots - 1) / slots) * (t + ak.count - 1);
+        if (cost < best - 1e-9) { best = cost; tz = t; }
+    }
+    const int tz_env = env_int("NDL_LINE_TZ", 0);
+    if (tz_env > 0) tz = std::min(tz_env, la.Z);
+    la.tz = tz;
+    la.tiles_z = (la.Z + tz - 1) / tz;
+    return lblocks * la.tiles_z <= 0x7fffffffLL;
+}
+
 int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                  const ndl_axis_kernel* k, int32_t accum, int32_t rounding, hipStream_t s)
 {
+    {
+        LineArgs la;
+        int v = 1, d = 8;
+        if (plan_line(src, dst, naxes, axes, k, accum, la, v, d)) {
+            hipError_t e = launch_line_march(la, src->dtype, v, d, s);
+            if (e != hipSuccess) return hip_fail(e, "line march launch");
+            return NDL_OK;
+        }
+    }
     FusedArgs fa;
     FusedConfig cfg;
     bool int_acc = false;

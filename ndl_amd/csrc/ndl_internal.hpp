@@ -228,6 +228,29 @@ struct EdgeTables {
     double* outd[3];  // double rows (or null)
 };
 hipError_t launch_edge_tables(const EdgeTables& e, hipStream_t s);
+// One-axis pass along a dimension other than the unit-stride one
+// (kernels_line.hip): every lane owns V consecutive x elements of one line
+// (x block, batch index) and marches along z with a register ring.
+// Workgroups: tiles_z x ceil(lanes / 256).
+struct LineArgs {
+    const void* src;
+    void* dst;
+    int32_t Z;               // convolved extent
+    int64_t sz, dz;          // src / dst strides of z (elements)
+    int32_t k, c, border;    // taps (odd count), centre, border mode
+    int32_t tz, tiles_z;     // outputs per chunk along z; chunks
+    uint32_t lanes;          // (X / V) x product of the batch extents
+    FastDiv xv;              // X / V lanes per line
+    int32_t nb;              // batch dimensions, fastest first
+    FastDiv bext[NDL_MAX_DIM];
+    int64_t sb[NDL_MAX_DIM];
+    int64_t db[NDL_MAX_DIM];
+    float w[NDL_MAX_TAPS];
+};
+// v: x elements per lane (1, 2, 4); d: loads in flight per lane
+hipError_t launch_line_march(const LineArgs& a, int dt, int v, int d, hipStream_t s);
+bool line_march_supported(int dt, int k, int v, int d);
+int line_march_blocks_per_cu(int dt, int k, int v, int d);
 // resident workgroups per CU for this instantiation (0 if unknown)
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
 // whether an instantiation exists (queue: the persistent work-queue kernel)

@@ -467,6 +467,46 @@ def test_row_march_2d_vs_oracle(border, shape, monkeypatch):
     assert rel_err(tile, want) <= 1e-6
 
 
+LINE_CASES = [
+    # (numpy shape, ndl axis, taps, view, line env) -- ndl axis 0 is the last numpy axis
+    ((40, 37, 300), 1, 11, None, {}),                       # y pass, partial x block
+    ((101, 9, 64), 2, 19, None, {}),                        # z pass, X at the minimum
+    ((101, 9, 64), 2, 19, None, {"NDL_LINE_TZ": "16"}),     # partial last z chunk
+    ((101, 9, 64), 2, 25, None, {"NDL_LINE_D": "4"}),       # shallow prefetch
+    ((3, 12, 5, 70), 2, 25, None, {"NDL_LINE_TZ": "5"}),    # reach = extent 12; two batch dims
+    ((3, 12, 5, 70), 3, 3, None, {}),                       # outermost axis
+    ((30, 20, 260), 2, 11, np.s_[::-1, :, :], {}),          # mirrored z (negative stride)
+    ((30, 20, 260), 1, 7, np.s_[:, ::3, :], {"NDL_LINE_TZ": "2"}),  # decimated y
+    ((20, 11, 67), 1, 9, None, {}),                         # odd x extent: one element per lane
+    ((30, 20, 260), 2, 5, np.s_[:, :, 1:-3], {}),           # x view off 16-byte alignment
+    ((101, 9, 64), 2, 19, None, {"NDL_LINE_V": "2"}),       # two elements per lane
+    ((101, 9, 64), 2, 13, None, {"NDL_LINE_D": "4"}),       # four per lane, shallow prefetch
+    ((64, 40, 256), 2, 11, None, {"NDL_LINE_TZ": "8"}),     # many chunks of many lane blocks
+]
+
+
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+@pytest.mark.parametrize("case", range(len(LINE_CASES)))
+def test_line_march_one_axis(case, border, monkeypatch):
+    # one float32 axis other than x marches down its lines with a register
+    # ring; the tile path (NDL_LINE_MARCH=0) is checked on the same input
+    shape, axis, k, view, env = LINE_CASES[case]
+    say("line march", shape, axis, k, border, env)
+    rng = np.random.default_rng(100 + case)
+    base = rng.random(shape).astype(np.float32)
+    v = base[view] if view is not None else base
+    taps = [workloads.gaussian_taps(k / 6.0)[:k] if k > 3 else np.array([0.25, 0.5, 0.25])]
+    assert len(taps[0]) == k, len(taps[0])
+    want = oracle.sepconv(np.ascontiguousarray(v), [axis], taps, [border])
+    for name, val in env.items():
+        monkeypatch.setenv(name, val)
+    got = gpu_sep(base, [axis], taps, [border], F32, view=view)
+    assert rel_err(got, want) <= 1e-6
+    monkeypatch.setenv("NDL_LINE_MARCH", "0")
+    tile = gpu_sep(base, [axis], taps, [border], F32, view=view)
+    assert rel_err(tile, want) <= 1e-6
+
+
 # ------------------------------------------------------- half-precision storage
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
 @pytest.mark.parametrize("border", ["clamp", "reflect"])
