@@ -299,7 +299,7 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
                   "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
                   "GBps_alg": round(2 * x.size / ms / 1e6, 1), "bit_exact": bool(ok)})
         # the 256^3 Gaussian's axes one at a time (convolve() with a 1-D
-        # kernel image per axis), cold cache: x on the tile kernel, y and z
+        # kernel image per axis), cold cache: x on the row kernel, y and z
         # on the line march
         x, axes, taps, borders = workloads.config2a()
         s = ndl_amd.DeviceArray.from_numpy(x)

@@ -601,6 +601,54 @@ bool plan_line(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, cons
     return lblocks * la.tiles_z <= 0x7fffffffLL;
 }
 
+// One float32 pass along the unit-stride axis, FP32 accumulation: the row
+// kernel (kernels_line.hip).  v = outputs per lane; blocks = grid size.
+bool plan_row(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+              const ndl_axis_kernel* k, int accum, RowArgs& ra, int& v, uint32_t& blocks)
+{
+    if (naxes != 1 || accum != NDL_ACCUM_F32 || !env_int("NDL_ROW_PASS", 1)) return false;
+    if (src->dtype != NDL_F32 || dst->dtype != NDL_F32) return false;
+    const ndl_axis_kernel& ak = k[0];
+    for (int t = 0; t < ak.count; t++)
+        if (ak.taps[t] == 0.0) return false;  // 0*inf would be NaN (detail.h:164-174)
+    const int nd = src->ndim, xd = axes[0];
+    if (src->stride[xd] != 1 || dst->stride[xd] != 1 || src->extent[xd] < 32 || src->extent[xd] > 0x7fffffff)
+        return false;
+    auto fits = [&](int vv) {
+        if (src->extent[xd] % vv || (uintptr_t)src->data % (4 * vv) || (uintptr_t)dst->data % (4 * vv)) return false;
+        for (int e = 0; e < nd; e++)
+            if (e != xd && src->extent[e] > 1 && (src->stride[e] % vv || dst->stride[e] % vv)) return false;
+        return true;
+    };
+    v = env_int("NDL_ROW_V", 4) == 1 || !fits(4) ? 1 : 4;
+    if (!row_pass_supported(NDL_F32, ak.count, v)) return false;
+    memset(&ra, 0, sizeof ra);
+    ra.src = src->data;
+    ra.dst = dst->data;
+    ra.X = (int32_t)src->extent[xd];
+    ra.k = ak.count;
+    ra.c = ak.center;
+    ra.border = ak.border;
+    for (int t = 0; t < ak.count; t++) ra.w[t] = (float)ak.taps[t];
+    const int64_t segs = (ra.X + 64 * v - 1) / (64 * v);
+    ra.segs.init((uint32_t)segs);
+    int64_t total = segs;
+    for (int e = 0; e < nd; e++) {
+        if (e == xd || src->extent[e] == 1) continue;
+        if (src->extent[e] > 0xffffffffLL) return false;
+        ra.bext[ra.nb].init((uint32_t)src->extent[e]);
+        ra.sb[ra.nb] = src->stride[e];
+        ra.db[ra.nb] = dst->stride[e];
+        total *= src->extent[e];
+        ra.nb++;
+        if (total > 0x7fff0000LL) return false;
+    }
+    ra.total = (uint32_t)total;
+    const int64_t slots = (int64_t)device_cus() * std::max(1, row_pass_blocks_per_cu(NDL_F32, ak.count, v));
+    blocks = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((total + 3) / 4, slots));
+    return true;
+}
+
 int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                  const ndl_axis_kernel* k, int32_t accum, int32_t rounding, hipStream_t s)
 {
@@ -610,6 +658,13 @@ int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, co
         if (plan_line(src, dst, naxes, axes, k, accum, la, v, d)) {
             hipError_t e = launch_line_march(la, src->dtype, v, d, s);
             if (e != hipSuccess) return hip_fail(e, "line march launch");
+            return NDL_OK;
+        }
+        RowArgs ra;
+        uint32_t blocks = 0;
+        if (plan_row(src, dst, naxes, axes, k, accum, ra, v, blocks)) {
+            hipError_t e = launch_row_pass(ra, src->dtype, v, blocks, s);
+            if (e != hipSuccess) return hip_fail(e, "row pass launch");
             return NDL_OK;
         }
     }

@@ -94,6 +94,113 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
     }
 }
 
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rows: every wave takes segments of L = 64 V consecutive outputs of one row
+// in turn (segment s, s + waves, ...).  A segment's inputs -- V elements per
+// lane plus the c left and K-1-c right halo elements, border-folded near the
+// row ends (single-fold maps, mathHelpers.h:23-25) -- go to the wave's own
+// LDS window; each lane then reads its K+V-1 window values and writes V
+// outputs.  The next segment's loads are issued before the current one is
+// computed.  Only wave-level ordering is needed (no workgroup barrier).
+template <int K, int V>
+__global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
+{
+    constexpr int L = 64 * V;
+    constexpr int HLMAX = (K - 1 + 3) & ~3;            // >= the aligned left halo for any centre
+    constexpr int W = (HLMAX + L + K + 3) & ~3;         // window floats per wave
+    __shared__ __align__(16) float win[4][W];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    float* buf = win[wv];
+    const int c = a.c;
+    const int hl = (c + 3) & ~3;                        // main part starts 16-byte aligned
+    const uint32_t nwaves = gridDim.x * 4u;
+
+    auto row_of = [&](uint32_t s, int64_t& so, int64_t& dof, int& x0) {
+        uint32_t r = a.segs.div(s);
+        x0 = (int)(s - r * a.segs.d) * L;
+        so = 0;
+        dof = 0;
+        for (int i = 0; i < a.nb; i++) {
+            const uint32_t q = a.bext[i].div(r);
+            const uint32_t rr = r - q * a.bext[i].d;
+            so += (int64_t)rr * a.sb[i];
+            dof += (int64_t)rr * a.db[i];
+            r = q;
+        }
+    };
+    auto issue = [&](uint32_t s, float* m, float& h) {
+        int64_t so, dof;
+        int x0;
+        row_of(s, so, dof, x0);
+        const float* row = (const float*)a.src + so;
+        const int xm = x0 + V * lane;
+        if (xm + V - 1 < a.X) {
+            load_v<V>(row + xm, m);
+        } else {
+#pragma unroll
+            for (int e = 0; e < V; e++) m[e] = row[safe_map32(a.border, a.X, xm + e)];
+        }
+        if (lane < K - 1) {
+            const int p = lane < c ? x0 - c + lane : x0 + L + (lane - c);
+            h = row[safe_map32(a.border, a.X, p)];
+        }
+    };
+
+    uint32_t seg = blockIdx.x * 4u + (uint32_t)wv;
+    float m[V], h = 0.f;
+    if (seg < a.total) issue(seg, m, h);
+    for (; seg < a.total; seg += nwaves) {
+        wave_lds_sync();  // the previous segment's window reads are done
+        store_v<V>(buf + hl + V * lane, m);
+        if (lane < K - 1) buf[lane < c ? hl - c + lane : hl + L + (lane - c)] = h;
+        wave_lds_sync();
+        if (seg + nwaves < a.total) issue(seg + nwaves, m, h);
+        float wv_[K + V - 1];
+#pragma unroll
+        for (int j = 0; j < K + V - 1; j++) wv_[j] = buf[hl - c + V * lane + j];
+        float acc[V];
+#pragma unroll
+        for (int e = 0; e < V; e++) acc[e] = 0.f;
+#pragma unroll
+        for (int t = 0; t < K; t++)
+#pragma unroll
+            for (int e = 0; e < V; e++) acc[e] = fmaf(a.w[t], wv_[e + t], acc[e]);
+        int64_t so, dof;
+        int x0;
+        row_of(seg, so, dof, x0);
+        float* drow = (float*)a.dst + dof;
+        const int xo = x0 + V * lane;
+        if (xo + V - 1 < a.X) {
+            store_v<V>(drow + xo, acc);
+        } else {
+#pragma unroll
+            for (int e = 0; e < V; e++)
+                if (xo + e < a.X) drow[xo + e] = acc[e];
+        }
+    }
+}
+
+template <class F> hipError_t with_row_instance(int dt, int k, int v, F&& f)
+{
+    if (dt != NDL_F32 || (v != 1 && v != 4)) return hipErrorInvalidValue;
+#define NDL_ROWK(KK)                                      \
+    if (k == KK) {                                        \
+        if (v == 4) return f(IC<KK>{}, IC<4>{});          \
+        return f(IC<KK>{}, IC<1>{});                      \
+    }
+    NDL_ROWK(3) NDL_ROWK(5) NDL_ROWK(7) NDL_ROWK(9) NDL_ROWK(11) NDL_ROWK(13) NDL_ROWK(15) NDL_ROWK(17)
+    NDL_ROWK(19) NDL_ROWK(21) NDL_ROWK(23) NDL_ROWK(25)
+#undef NDL_ROWK
+    return hipErrorInvalidValue;
+}
+
 // f(K, D, V) for the supported combinations (float32 storage)
 template <class F> hipError_t with_line_instance(int dt, int k, int v, int d, F&& f)
 {
@@ -130,6 +237,31 @@ int line_march_blocks_per_cu(int dt, int k, int v, int d)
             256, 0);
     });
     return e == hipSuccess ? n : 0;
+}
+
+bool row_pass_supported(int dt, int k, int v)
+{
+    return with_row_instance(dt, k, v, [](auto, auto) { return hipSuccess; }) == hipSuccess;
+}
+
+int row_pass_blocks_per_cu(int dt, int k, int v)
+{
+    int n = 0;
+    hipError_t e = with_row_instance(dt, k, v, [&](auto kk, auto vv) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(&row_pass_kernel<decltype(kk)::value, decltype(vv)::value>), 256, 0);
+    });
+    return e == hipSuccess ? n : 0;
+}
+
+hipError_t launch_row_pass(const RowArgs& a, int dt, int v, uint32_t blocks, hipStream_t s)
+{
+    if (a.total == 0 || blocks == 0) return hipSuccess;
+    return with_row_instance(dt, a.k, v, [&](auto kk, auto vv) {
+        hipLaunchKernelGGL((row_pass_kernel<decltype(kk)::value, decltype(vv)::value>), dim3(blocks), dim3(256), 0, s,
+                           a);
+        return hipGetLastError();
+    });
 }
 
 hipError_t launch_line_march(const LineArgs& a, int dt, int v, int d, hipStream_t s)

@@ -251,6 +251,25 @@ struct LineArgs {
 hipError_t launch_line_march(const LineArgs& a, int dt, int v, int d, hipStream_t s);
 bool line_march_supported(int dt, int k, int v, int d);
 int line_march_blocks_per_cu(int dt, int k, int v, int d);
+// One-axis pass along the unit-stride dimension (kernels_line.hip): a wave
+// takes a segment of 64 x V outputs of one row (a fixed index of every other
+// dimension) and its halo through a wave-private LDS window.
+struct RowArgs {
+    const void* src;
+    void* dst;
+    int32_t X;               // row length
+    int32_t k, c, border;    // taps (odd count), centre, border mode
+    uint32_t total;          // rows x segments per row
+    FastDiv segs;            // segments per row, ceil(X / (64 V))
+    int32_t nb;              // row dimensions, fastest first
+    FastDiv bext[NDL_MAX_DIM];
+    int64_t sb[NDL_MAX_DIM];
+    int64_t db[NDL_MAX_DIM];
+    float w[NDL_MAX_TAPS];
+};
+hipError_t launch_row_pass(const RowArgs& a, int dt, int v, uint32_t blocks, hipStream_t s);
+bool row_pass_supported(int dt, int k, int v);
+int row_pass_blocks_per_cu(int dt, int k, int v);
 // resident workgroups per CU for this instantiation (0 if unknown)
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
 // whether an instantiation exists (queue: the persistent work-queue kernel)

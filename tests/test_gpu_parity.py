@@ -507,6 +507,42 @@ def test_line_march_one_axis(case, border, monkeypatch):
     assert rel_err(tile, want) <= 1e-6
 
 
+ROW_CASES = [
+    # (numpy shape, taps, centre or None, view, env): a pass along x, the unit-stride axis
+    ((40, 37, 300), 11, None, None, {}),                    # partial second segment
+    ((20, 11, 67), 9, None, None, {}),                      # odd row: one output per lane
+    ((12, 5, 32), 25, None, None, {}),                      # reach 12 of a 32-element row
+    ((7, 300), 7, 0, None, {}),                             # every halo element on the right
+    ((7, 300), 7, 6, None, {}),                             # every halo element on the left
+    ((9, 300), 13, 3, None, {"NDL_ROW_V": "1"}),
+    ((30, 20, 260), 5, None, np.s_[:, ::-1, :], {}),        # mirrored rows
+    ((30, 20, 260), 19, None, np.s_[:, :, 1:-3], {}),       # x view off 16-byte alignment
+    ((3, 600, 1100), 21, None, None, {}),                   # many segments per wave
+]
+
+
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+@pytest.mark.parametrize("case", range(len(ROW_CASES)))
+def test_row_pass_x_axis(case, border, monkeypatch):
+    # one float32 pass along x: segments of a row through a wave's LDS
+    # window; the tile path (NDL_ROW_PASS=0) is checked on the same input
+    shape, k, centre, view, env = ROW_CASES[case]
+    say("row pass", shape, k, centre, border, env)
+    rng = np.random.default_rng(200 + case)
+    base = rng.random(shape).astype(np.float32)
+    v = base[view] if view is not None else base
+    taps = [rng.random(k) + 0.05]
+    centers = None if centre is None else [centre]
+    want = oracle.sepconv(np.ascontiguousarray(v), [0], taps, [border], centers=centers)
+    for name, val in env.items():
+        monkeypatch.setenv(name, val)
+    got = gpu_sep(base, [0], taps, [border], F32, centers=centers, view=view)
+    assert rel_err(got, want) <= 1e-6
+    monkeypatch.setenv("NDL_ROW_PASS", "0")
+    tile = gpu_sep(base, [0], taps, [border], F32, centers=centers, view=view)
+    assert rel_err(tile, want) <= 1e-6
+
+
 # ------------------------------------------------------- half-precision storage
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
 @pytest.mark.parametrize("border", ["clamp", "reflect"])
