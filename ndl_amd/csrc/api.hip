@@ -209,6 +209,9 @@ int run_axis_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const n
     return NDL_OK;
 }
 
+int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
+                 hipStream_t s);
+
 // per-axis chain through dense accumulator-typed intermediates
 int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                       const ndl_axis_kernel* k, int accum, int rounding, hipStream_t s)
@@ -228,8 +231,10 @@ int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxe
     for (int i = 0; i < naxes; i++) {
         bool last = i == naxes - 1;
         const ndl_tensor* out = last ? dst : &mid[i & 1];
-        int rc = run_axis_pass(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC,
-                               PASS_GATHER, nullptr, 0, 0, s);
+        int rc = try_one_axis(cur, out, axes[i], k[i], accum, s);
+        if (rc == -1)
+            rc = run_axis_pass(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, PASS_GATHER,
+                               nullptr, 0, 0, s);
         if (rc) return rc;
         cur = out;
     }
@@ -528,56 +533,81 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     return true;
 }
 
-// One float32 axis that is not the unit-stride one, FP32 accumulation:
-// the line march (kernels_line.hip).  v = x elements per lane, d = loads in
-// flight per lane.
-bool plan_line(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
-               const ndl_axis_kernel* k, int accum, LineArgs& la, int& v, int& d)
+// One-axis kernels (kernels_line.hip): float32 / float64 storage, FP32
+// accumulation (float in and out) or the reference's FP64 arithmetic, odd
+// tap counts up to 25 and no zero tap (the reference skips those,
+// detail.h:164-174; the per-axis pass handles them).
+bool one_axis_types(const ndl_tensor* in, const ndl_tensor* out, const ndl_axis_kernel& ak, int accum,
+                    LineKey& key)
 {
-    if (naxes != 1 || accum != NDL_ACCUM_F32 || !env_int("NDL_LINE_MARCH", 1)) return false;
-    const int dt = src->dtype;
-    if (dt != NDL_F32 || dst->dtype != NDL_F32) return false;
-    const ndl_axis_kernel& ak = k[0];
-    // a zero tap must not multiply (detail.h:164-174: 0*inf would be NaN)
+    const int it = in->dtype, ot = out->dtype;
+    if ((it != NDL_F32 && it != NDL_F64) || (ot != NDL_F32 && ot != NDL_F64)) return false;
+    if (accum == NDL_ACCUM_F32 ? (it != NDL_F32 || ot != NDL_F32) : accum != NDL_ACCUM_REF) return false;
     for (int t = 0; t < ak.count; t++)
         if (ak.taps[t] == 0.0) return false;
-    const int nd = src->ndim, zd = axes[0];
-    int xd = -1;
-    for (int e = 0; e < nd; e++)
-        if (src->stride[e] == 1 && dst->stride[e] == 1 && src->extent[e] > 1) { xd = e; break; }
-    if (xd < 0 || xd == zd || src->extent[xd] < 64 || src->extent[zd] > 0x7fffffff) return false;
-    // widest vector whose every lane address is aligned: x extent, base
-    // addresses and every other stride multiples of v elements
-    auto fits = [&](int vv) {
-        if (src->extent[xd] % vv || (uintptr_t)src->data % (4 * vv) || (uintptr_t)dst->data % (4 * vv)) return false;
-        for (int e = 0; e < nd; e++)
-            if (e != xd && src->extent[e] > 1 && (src->stride[e] % vv || dst->stride[e] % vv)) return false;
-        return true;
-    };
-    v = env_int("NDL_LINE_V", 4);
-    if (v != 1 && v != 2 && v != 4) v = 4;
-    while (v > 1 && !fits(v)) v /= 2;
-    d = env_int("NDL_LINE_D", 8);
-    if (!line_march_supported(dt, ak.count, v, d)) return false;
+    key = LineKey{it, ot, accum, ak.count, 1, 8};
+    return true;
+}
+
+// unit-stride dimension of both tensors (-1: none)
+int unit_dim(const ndl_tensor* in, const ndl_tensor* out)
+{
+    for (int e = 0; e < in->ndim; e++)
+        if (in->stride[e] == 1 && out->stride[e] == 1 && in->extent[e] > 1) return e;
+    return -1;
+}
+
+// every lane address aligned for v elements: x extent, base addresses and
+// every other stride multiples of v
+bool vec_fits(const ndl_tensor* in, const ndl_tensor* out, int xd, int v)
+{
+    if (in->extent[xd] % v || (uintptr_t)in->data % (dtype_size(in->dtype) * v) ||
+        (uintptr_t)out->data % (dtype_size(out->dtype) * v))
+        return false;
+    for (int e = 0; e < in->ndim; e++)
+        if (e != xd && in->extent[e] > 1 && (in->stride[e] % v || out->stride[e] % v)) return false;
+    return true;
+}
+
+// An axis other than the unit-stride one: the line march.
+bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_axis_kernel& ak, int accum,
+               LineArgs& la, LineKey& key)
+{
+    if (!env_int("NDL_LINE_MARCH", 1) || !one_axis_types(in, out, ak, accum, key)) return false;
+    const int nd = in->ndim, xd = unit_dim(in, out);
+    if (xd < 0 || xd == zd || in->extent[xd] < 64 || in->extent[zd] > 0x7fffffff) return false;
+    // the widest lanes (16 bytes at most, NDL_LINE_V caps them) whose
+    // addresses are aligned and that have an instance
+    int v = std::min(16 / (int)dtype_size(in->dtype), env_int("NDL_LINE_V", 4));
+    v = v >= 4 ? 4 : (v >= 2 ? 2 : 1);
+    key.d = env_int("NDL_LINE_D", 8);
+    for (;; v /= 2) {
+        key.v = v;
+        if (vec_fits(in, out, xd, v) && line_march_supported(key)) break;
+        if (v == 1) return false;
+    }
     memset(&la, 0, sizeof la);
-    la.src = src->data;
-    la.dst = dst->data;
-    la.Z = (int32_t)src->extent[zd];
-    la.sz = src->stride[zd];
-    la.dz = dst->stride[zd];
+    la.src = in->data;
+    la.dst = out->data;
+    la.Z = (int32_t)in->extent[zd];
+    la.sz = in->stride[zd];
+    la.dz = out->stride[zd];
     la.k = ak.count;
     la.c = ak.center;
     la.border = ak.border;
-    for (int t = 0; t < ak.count; t++) la.w[t] = (float)ak.taps[t];
-    const int64_t xv = src->extent[xd] / v;
+    for (int t = 0; t < ak.count; t++) {
+        la.w[t] = (float)ak.taps[t];
+        la.wd[t] = ak.taps[t];
+    }
+    const int64_t xv = in->extent[xd] / v;
     int64_t lanes = xv;
     la.xv.init((uint32_t)xv);
     for (int e = 0; e < nd; e++) {
-        if (e == xd || e == zd || src->extent[e] == 1) continue;
-        la.bext[la.nb].init((uint32_t)std::min<int64_t>(src->extent[e], 0xffffffffLL));
-        la.sb[la.nb] = src->stride[e];
-        la.db[la.nb] = dst->stride[e];
-        lanes *= src->extent[e];
+        if (e == xd || e == zd || in->extent[e] == 1) continue;
+        la.bext[la.nb].init((uint32_t)std::min<int64_t>(in->extent[e], 0xffffffffLL));
+        la.sb[la.nb] = in->stride[e];
+        la.db[la.nb] = out->stride[e];
+        lanes *= in->extent[e];
         la.nb++;
         if (lanes > 0xfffffe00LL) return false;
     }
@@ -585,7 +615,7 @@ bool plan_line(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, cons
     // z chunks: fill the resident workgroup slots in whole rounds; a chunk
     // of tz outputs reads tz + k - 1 slices
     const int64_t lblocks = (lanes + 255) / 256;
-    const int64_t slots = (int64_t)device_cus() * std::max(1, line_march_blocks_per_cu(dt, ak.count, v, d));
+    const int64_t slots = (int64_t)device_cus() * std::max(1, line_march_blocks_per_cu(key));
     int tz = la.Z;
     double best = 1e300;
     for (int chunks = 1; chunks <= std::min(la.Z, 256); chunks++) {
@@ -601,72 +631,72 @@ bool plan_line(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, cons
     return lblocks * la.tiles_z <= 0x7fffffffLL;
 }
 
-// One float32 pass along the unit-stride axis, FP32 accumulation: the row
-// kernel (kernels_line.hip).  v = outputs per lane; blocks = grid size.
-bool plan_row(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
-              const ndl_axis_kernel* k, int accum, RowArgs& ra, int& v, uint32_t& blocks)
+// The unit-stride axis: the row kernel.  blocks = grid size.
+bool plan_row(const ndl_tensor* in, const ndl_tensor* out, int xd, const ndl_axis_kernel& ak, int accum,
+              RowArgs& ra, LineKey& key, uint32_t& blocks)
 {
-    if (naxes != 1 || accum != NDL_ACCUM_F32 || !env_int("NDL_ROW_PASS", 1)) return false;
-    if (src->dtype != NDL_F32 || dst->dtype != NDL_F32) return false;
-    const ndl_axis_kernel& ak = k[0];
-    for (int t = 0; t < ak.count; t++)
-        if (ak.taps[t] == 0.0) return false;  // 0*inf would be NaN (detail.h:164-174)
-    const int nd = src->ndim, xd = axes[0];
-    if (src->stride[xd] != 1 || dst->stride[xd] != 1 || src->extent[xd] < 32 || src->extent[xd] > 0x7fffffff)
+    if (!env_int("NDL_ROW_PASS", 1) || !one_axis_types(in, out, ak, accum, key)) return false;
+    const int nd = in->ndim;
+    if (in->stride[xd] != 1 || out->stride[xd] != 1 || in->extent[xd] < 32 || in->extent[xd] > 0x7fffffff)
         return false;
-    auto fits = [&](int vv) {
-        if (src->extent[xd] % vv || (uintptr_t)src->data % (4 * vv) || (uintptr_t)dst->data % (4 * vv)) return false;
-        for (int e = 0; e < nd; e++)
-            if (e != xd && src->extent[e] > 1 && (src->stride[e] % vv || dst->stride[e] % vv)) return false;
-        return true;
-    };
-    v = env_int("NDL_ROW_V", 4) == 1 || !fits(4) ? 1 : 4;
-    if (!row_pass_supported(NDL_F32, ak.count, v)) return false;
+    key.v = env_int("NDL_ROW_V", 4) == 1 || !vec_fits(in, out, xd, 4) ? 1 : 4;
+    if (!row_pass_supported(key)) return false;
     memset(&ra, 0, sizeof ra);
-    ra.src = src->data;
-    ra.dst = dst->data;
-    ra.X = (int32_t)src->extent[xd];
+    ra.src = in->data;
+    ra.dst = out->data;
+    ra.X = (int32_t)in->extent[xd];
     ra.k = ak.count;
     ra.c = ak.center;
     ra.border = ak.border;
-    for (int t = 0; t < ak.count; t++) ra.w[t] = (float)ak.taps[t];
-    const int64_t segs = (ra.X + 64 * v - 1) / (64 * v);
+    for (int t = 0; t < ak.count; t++) {
+        ra.w[t] = (float)ak.taps[t];
+        ra.wd[t] = ak.taps[t];
+    }
+    const int64_t segs = (ra.X + 64 * key.v - 1) / (64 * key.v);
     ra.segs.init((uint32_t)segs);
     int64_t total = segs;
     for (int e = 0; e < nd; e++) {
-        if (e == xd || src->extent[e] == 1) continue;
-        if (src->extent[e] > 0xffffffffLL) return false;
-        ra.bext[ra.nb].init((uint32_t)src->extent[e]);
-        ra.sb[ra.nb] = src->stride[e];
-        ra.db[ra.nb] = dst->stride[e];
-        total *= src->extent[e];
+        if (e == xd || in->extent[e] == 1) continue;
+        if (in->extent[e] > 0xffffffffLL) return false;
+        ra.bext[ra.nb].init((uint32_t)in->extent[e]);
+        ra.sb[ra.nb] = in->stride[e];
+        ra.db[ra.nb] = out->stride[e];
+        total *= in->extent[e];
         ra.nb++;
         if (total > 0x7fff0000LL) return false;
     }
     ra.total = (uint32_t)total;
-    const int64_t slots = (int64_t)device_cus() * std::max(1, row_pass_blocks_per_cu(NDL_F32, ak.count, v));
+    const int64_t slots = (int64_t)device_cus() * std::max(1, row_pass_blocks_per_cu(key));
     blocks = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((total + 3) / 4, slots));
     return true;
+}
+
+// One axis through the line march or the row kernel: NDL_OK when it ran,
+// -1 when neither applies, an error code otherwise.
+int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
+                 hipStream_t s)
+{
+    LineKey key;
+    LineArgs la;
+    if (plan_line(in, out, axis, ak, accum, la, key)) {
+        hipError_t e = launch_line_march(la, key, s);
+        return e == hipSuccess ? NDL_OK : hip_fail(e, "line march launch");
+    }
+    RowArgs ra;
+    uint32_t blocks = 0;
+    if (plan_row(in, out, axis, ak, accum, ra, key, blocks)) {
+        hipError_t e = launch_row_pass(ra, key, blocks, s);
+        return e == hipSuccess ? NDL_OK : hip_fail(e, "row pass launch");
+    }
+    return -1;
 }
 
 int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                  const ndl_axis_kernel* k, int32_t accum, int32_t rounding, hipStream_t s)
 {
-    {
-        LineArgs la;
-        int v = 1, d = 8;
-        if (plan_line(src, dst, naxes, axes, k, accum, la, v, d)) {
-            hipError_t e = launch_line_march(la, src->dtype, v, d, s);
-            if (e != hipSuccess) return hip_fail(e, "line march launch");
-            return NDL_OK;
-        }
-        RowArgs ra;
-        uint32_t blocks = 0;
-        if (plan_row(src, dst, naxes, axes, k, accum, ra, v, blocks)) {
-            hipError_t e = launch_row_pass(ra, src->dtype, v, blocks, s);
-            if (e != hipSuccess) return hip_fail(e, "row pass launch");
-            return NDL_OK;
-        }
+    if (naxes == 1) {
+        const int rc = try_one_axis(src, dst, axes[0], k[0], accum, s);
+        if (rc != -1) return rc;
     }
     FusedArgs fa;
     FusedConfig cfg;

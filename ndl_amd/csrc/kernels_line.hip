@@ -1,17 +1,27 @@
-// kernels_line.hip -- one-axis passes along a dimension that is not the
-// unit-stride one (a 1-D kernel image {1, k, 1} or {1, 1, k} given to
-// convolve(), processing/convolution.h:52-71, or one per-axis pass of
-// gaussian_blur(), convolution.h:107-186).
+// kernels_line.hip -- one-axis passes over float32 / float64 storage: a 1-D
+// kernel image {k, 1, 1}, {1, k, 1}, ... given to convolve()
+// (processing/convolution.h:52-71), or one per-axis pass of gaussian_blur()
+// (convolution.h:107-186, through FP64 intermediates).
 //
-// Every lane owns V consecutive elements of the unit-stride dimension x of
-// one line (a fixed index of every other dimension) and marches along the
-// convolved axis z.  The input slices it has read sit in a ring of K + D
-// registers per element: slice i+D is requested while output i-(K-1) is
-// computed, so D loads per lane stay in flight.  z positions, their border
-// fold and the taps are the same for the whole workgroup (scalar registers).
-// Lanes are numbered x-fastest over (x / V, line), so a workgroup reads
-// 256 x V x 4 contiguous bytes per slice where x rows are dense.  No LDS and
-// no barrier.
+// Arithmetic (this file is built with -ffp-contract=off):
+//  * Acc = double (NDL_ACCUM_REF): total = 0, then total = total +
+//    double(x) * tap for the taps in ascending order, every product and sum
+//    rounded separately, one rounding into the output type -- the
+//    reference's loop exactly.  Callers send no zero tap here (the reference
+//    skips them, detail.h:164-174).
+//  * Acc = float (NDL_ACCUM_F32): acc = fmaf(tap, x, acc), taps ascending.
+//
+// line_march_kernel (an axis other than the unit-stride one): every lane owns
+// V consecutive elements of the unit-stride dimension x of one line (a fixed
+// index of every other dimension) and marches along the convolved axis z.
+// The input slices it has read sit in a ring of K + D registers per element:
+// slice i+D is requested while output i-(K-1) is computed, so D loads per
+// lane stay in flight.  z positions, their border fold and the taps are the
+// same for the whole workgroup (scalar registers).  Lanes are numbered
+// x-fastest over (x / V, line), so a workgroup reads 256 x V contiguous
+// elements per slice where x rows are dense.  No LDS and no barrier.
+//
+// row_pass_kernel (the unit-stride axis): see below.
 #include "ndl_internal.hpp"
 
 namespace ndl {
@@ -19,26 +29,36 @@ namespace {
 
 template <int V> using IC = std::integral_constant<int, V>;
 
-template <int V> __device__ __forceinline__ void load_v(const float* p, float* v)
+// V consecutive elements in one access (V * sizeof(T) bytes, aligned to it)
+template <class T, int V> struct alignas(sizeof(T) * V) Vec {
+    T v[V];
+};
+template <class T, int V> __device__ __forceinline__ void load_v(const T* p, T* v)
 {
-    if constexpr (V == 4) {
-        const float4 q = *reinterpret_cast<const float4*>(p);
-        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-    } else if constexpr (V == 2) {
-        const float2 q = *reinterpret_cast<const float2*>(p);
-        v[0] = q.x; v[1] = q.y;
-    } else {
-        v[0] = *p;
-    }
+    const Vec<T, V> q = *reinterpret_cast<const Vec<T, V>*>(p);
+#pragma unroll
+    for (int e = 0; e < V; e++) v[e] = q.v[e];
 }
-template <int V> __device__ __forceinline__ void store_v(float* p, const float* v)
+template <class T, int V> __device__ __forceinline__ void store_v(T* p, const T* v)
 {
-    if constexpr (V == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-    else if constexpr (V == 2) *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
-    else *p = v[0];
+    Vec<T, V> q;
+#pragma unroll
+    for (int e = 0; e < V; e++) q.v[e] = v[e];
+    *reinterpret_cast<Vec<T, V>*>(p) = q;
 }
 
-template <int K, int D, int V>
+template <class Acc> __device__ __forceinline__ Acc mac1(Acc acc, Acc w, Acc x);
+template <> __device__ __forceinline__ double mac1<double>(double acc, double w, double x)
+{
+    return acc + x * w; // two roundings, as the reference's `total += src * k`
+}
+template <> __device__ __forceinline__ float mac1<float>(float acc, float w, float x) { return fmaf(w, x, acc); }
+
+template <class Acc> __device__ __forceinline__ Acc tap(const float* w, const double* wd, int t);
+template <> __device__ __forceinline__ double tap<double>(const float*, const double* wd, int t) { return wd[t]; }
+template <> __device__ __forceinline__ float tap<float>(const float* w, const double*, int t) { return w[t]; }
+
+template <class TIn, class TOut, class Acc, int K, int D, int V>
 __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
 {
     constexpr int R = K + D;  // ring: K inputs of the current output plus D in flight
@@ -60,17 +80,17 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
         dof += (int64_t)r * a.db[i];
         b = q;
     }
-    const float* scol = (const float*)a.src + so;
-    float* dcol = (float*)a.dst + dof;
+    const TIn* scol = (const TIn*)a.src + so;
+    TOut* dcol = (TOut*)a.dst + dof;
     const int z0 = (int)chunk * a.tz;
     const int n_out = min(a.tz, a.Z - z0);
     const int n_in = n_out + K - 1;
     const int p0 = z0 - a.c;  // position of input 0
 
-    float ring[R][V];
-    auto load = [&](int i, float* v) {
+    TIn ring[R][V];
+    auto load = [&](int i, TIn* v) {
         const int p = safe_map32(a.border, a.Z, p0 + i);
-        load_v<V>(scol + (int64_t)p * a.sz, v);
+        load_v<TIn, V>(scol + (int64_t)p * a.sz, v);
     };
 #pragma unroll
     for (int j = 0; j < D; j++)
@@ -81,14 +101,20 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
             const int i = i0 + u;
             if (i + D < n_in) load(i + D, ring[(u + D) % R]);
             if (i >= K - 1 && i < n_in) {
-                float acc[V];
+                Acc acc[V];
 #pragma unroll
-                for (int e = 0; e < V; e++) acc[e] = 0.f;
+                for (int e = 0; e < V; e++) acc[e] = 0;
 #pragma unroll
                 for (int t = 0; t < K; t++)
 #pragma unroll
-                    for (int e = 0; e < V; e++) acc[e] = fmaf(a.w[t], ring[(u - (K - 1) + t + R) % R][e], acc[e]);
-                if (live) store_v<V>(dcol + (int64_t)(z0 + i - (K - 1)) * a.dz, acc);
+                    for (int e = 0; e < V; e++)
+                        acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), (Acc)ring[(u - (K - 1) + t + R) % R][e]);
+                if (live) {
+                    TOut o[V];
+#pragma unroll
+                    for (int e = 0; e < V; e++) o[e] = (TOut)acc[e];
+                    store_v<TOut, V>(dcol + (int64_t)(z0 + i - (K - 1)) * a.dz, o);
+                }
             }
         }
     }
@@ -108,18 +134,18 @@ __device__ __forceinline__ void wave_lds_sync()
 // LDS window; each lane then reads its K+V-1 window values and writes V
 // outputs.  The next segment's loads are issued before the current one is
 // computed.  Only wave-level ordering is needed (no workgroup barrier).
-template <int K, int V>
+template <class TIn, class TOut, class Acc, int K, int V>
 __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
 {
     constexpr int L = 64 * V;
     constexpr int HLMAX = (K - 1 + 3) & ~3;            // >= the aligned left halo for any centre
-    constexpr int W = (HLMAX + L + K + 3) & ~3;         // window floats per wave
-    __shared__ __align__(16) float win[4][W];
+    constexpr int W = (HLMAX + L + K + 3) & ~3;         // window elements per wave
+    __shared__ __align__(32) TIn win[4][W];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    float* buf = win[wv];
+    TIn* buf = win[wv];
     const int c = a.c;
-    const int hl = (c + 3) & ~3;                        // main part starts 16-byte aligned
+    const int hl = (c + 3) & ~3;                        // main part starts on a 4-element boundary
     const uint32_t nwaves = gridDim.x * 4u;
 
     auto row_of = [&](uint32_t s, int64_t& so, int64_t& dof, int& x0) {
@@ -135,14 +161,14 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
             r = q;
         }
     };
-    auto issue = [&](uint32_t s, float* m, float& h) {
+    auto issue = [&](uint32_t s, TIn* m, TIn& h) {
         int64_t so, dof;
         int x0;
         row_of(s, so, dof, x0);
-        const float* row = (const float*)a.src + so;
+        const TIn* row = (const TIn*)a.src + so;
         const int xm = x0 + V * lane;
         if (xm + V - 1 < a.X) {
-            load_v<V>(row + xm, m);
+            load_v<TIn, V>(row + xm, m);
         } else {
 #pragma unroll
             for (int e = 0; e < V; e++) m[e] = row[safe_map32(a.border, a.X, xm + e)];
@@ -154,122 +180,158 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
     };
 
     uint32_t seg = blockIdx.x * 4u + (uint32_t)wv;
-    float m[V], h = 0.f;
+    TIn m[V], h = 0;
     if (seg < a.total) issue(seg, m, h);
     for (; seg < a.total; seg += nwaves) {
         wave_lds_sync();  // the previous segment's window reads are done
-        store_v<V>(buf + hl + V * lane, m);
+        store_v<TIn, V>(buf + hl + V * lane, m);
         if (lane < K - 1) buf[lane < c ? hl - c + lane : hl + L + (lane - c)] = h;
         wave_lds_sync();
         if (seg + nwaves < a.total) issue(seg + nwaves, m, h);
-        float wv_[K + V - 1];
+        TIn wv_[K + V - 1];
 #pragma unroll
         for (int j = 0; j < K + V - 1; j++) wv_[j] = buf[hl - c + V * lane + j];
-        float acc[V];
+        Acc acc[V];
 #pragma unroll
-        for (int e = 0; e < V; e++) acc[e] = 0.f;
+        for (int e = 0; e < V; e++) acc[e] = 0;
 #pragma unroll
         for (int t = 0; t < K; t++)
 #pragma unroll
-            for (int e = 0; e < V; e++) acc[e] = fmaf(a.w[t], wv_[e + t], acc[e]);
+            for (int e = 0; e < V; e++) acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), (Acc)wv_[e + t]);
         int64_t so, dof;
         int x0;
         row_of(seg, so, dof, x0);
-        float* drow = (float*)a.dst + dof;
+        TOut* drow = (TOut*)a.dst + dof;
         const int xo = x0 + V * lane;
+        TOut o[V];
+#pragma unroll
+        for (int e = 0; e < V; e++) o[e] = (TOut)acc[e];
         if (xo + V - 1 < a.X) {
-            store_v<V>(drow + xo, acc);
+            store_v<TOut, V>(drow + xo, o);
         } else {
 #pragma unroll
             for (int e = 0; e < V; e++)
-                if (xo + e < a.X) drow[xo + e] = acc[e];
+                if (xo + e < a.X) drow[xo + e] = o[e];
         }
     }
 }
 
-template <class F> hipError_t with_row_instance(int dt, int k, int v, F&& f)
+// f(TIn, TOut, Acc) for the storage / accumulation combinations
+template <class F> hipError_t with_types(const LineKey& key, F&& f)
 {
-    if (dt != NDL_F32 || (v != 1 && v != 4)) return hipErrorInvalidValue;
-#define NDL_ROWK(KK)                                      \
-    if (k == KK) {                                        \
-        if (v == 4) return f(IC<KK>{}, IC<4>{});          \
-        return f(IC<KK>{}, IC<1>{});                      \
+    if (key.accum == NDL_ACCUM_F32) {
+        if (key.in_dt == NDL_F32 && key.out_dt == NDL_F32) return f(float{}, float{}, float{});
+        return hipErrorInvalidValue;
     }
-    NDL_ROWK(3) NDL_ROWK(5) NDL_ROWK(7) NDL_ROWK(9) NDL_ROWK(11) NDL_ROWK(13) NDL_ROWK(15) NDL_ROWK(17)
-    NDL_ROWK(19) NDL_ROWK(21) NDL_ROWK(23) NDL_ROWK(25)
-#undef NDL_ROWK
+    if (key.accum != NDL_ACCUM_REF) return hipErrorInvalidValue;
+    if (key.in_dt == NDL_F32 && key.out_dt == NDL_F32) return f(float{}, float{}, double{});
+    if (key.in_dt == NDL_F32 && key.out_dt == NDL_F64) return f(float{}, double{}, double{});
+    if (key.in_dt == NDL_F64 && key.out_dt == NDL_F64) return f(double{}, double{}, double{});
+    if (key.in_dt == NDL_F64 && key.out_dt == NDL_F32) return f(double{}, float{}, double{});
     return hipErrorInvalidValue;
 }
 
-// f(K, D, V) for the supported combinations (float32 storage)
-template <class F> hipError_t with_line_instance(int dt, int k, int v, int d, F&& f)
+template <class F> hipError_t with_k(int k, F&& f)
 {
-    if (dt != NDL_F32) return hipErrorInvalidValue;
-#define NDL_LINE(KK)                                                  \
-    if (k == KK) {                                                    \
-        if (v == 1 && d == 8) return f(IC<KK>{}, IC<8>{}, IC<1>{});   \
-        if (v == 2 && d == 8) return f(IC<KK>{}, IC<8>{}, IC<2>{});   \
-        if (v == 4 && d == 4) return f(IC<KK>{}, IC<4>{}, IC<4>{});   \
-        if (v == 4 && d == 8) return f(IC<KK>{}, IC<8>{}, IC<4>{});   \
-        return hipErrorInvalidValue;                                  \
+    switch (k) {
+#define NDL_K(KK) case KK: return f(IC<KK>{});
+        NDL_K(3) NDL_K(5) NDL_K(7) NDL_K(9) NDL_K(11) NDL_K(13) NDL_K(15) NDL_K(17) NDL_K(19) NDL_K(21) NDL_K(23)
+        NDL_K(25)
+#undef NDL_K
     }
-    NDL_LINE(3) NDL_LINE(5) NDL_LINE(7) NDL_LINE(9) NDL_LINE(11) NDL_LINE(13) NDL_LINE(15) NDL_LINE(17)
-    NDL_LINE(19) NDL_LINE(21) NDL_LINE(23) NDL_LINE(25)
-#undef NDL_LINE
     return hipErrorInvalidValue;
+}
+
+// f(TIn, TOut, Acc, K, D, V): line march instances.  Float input: V 4, 2, 1
+// with 8 loads in flight (and V 4 with 4 for FP32 accumulation; no V 4 with
+// FP64 accumulators beyond 21 taps, whose ring and sums would spill); double
+// input: V 2, 1 (16 bytes per lane at most).
+template <class F> hipError_t with_line_instance(const LineKey& key, F&& f)
+{
+    return with_types(key, [&](auto ti, auto to, auto ac) {
+        using TIn = decltype(ti);
+        return with_k(key.k, [&](auto kk) {
+            if constexpr (sizeof(TIn) == 4 && (sizeof(decltype(ac)) == 4 || decltype(kk)::value <= 21)) {
+                if (key.v == 4 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<4>{});
+                if constexpr (std::is_same_v<decltype(ac), float>)
+                    if (key.v == 4 && key.d == 4) return f(ti, to, ac, kk, IC<4>{}, IC<4>{});
+            }
+            if (key.v == 2 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<2>{});
+            if (key.v == 1 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<1>{});
+            return hipErrorInvalidValue;
+        });
+    });
+}
+
+// f(TIn, TOut, Acc, K, V): row instances, V 4 or 1
+template <class F> hipError_t with_row_instance(const LineKey& key, F&& f)
+{
+    return with_types(key, [&](auto ti, auto to, auto ac) {
+        return with_k(key.k, [&](auto kk) {
+            if (key.v == 4) return f(ti, to, ac, kk, IC<4>{});
+            if (key.v == 1) return f(ti, to, ac, kk, IC<1>{});
+            return hipErrorInvalidValue;
+        });
+    });
 }
 
 } // namespace
 
-bool line_march_supported(int dt, int k, int v, int d)
+bool row_pass_supported(const LineKey& key)
 {
-    return with_line_instance(dt, k, v, d, [](auto, auto, auto) { return hipSuccess; }) == hipSuccess;
+    return with_row_instance(key, [](auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
 }
 
-int line_march_blocks_per_cu(int dt, int k, int v, int d)
+int row_pass_blocks_per_cu(const LineKey& key)
 {
     int n = 0;
-    hipError_t e = with_line_instance(dt, k, v, d, [&](auto kk, auto dd, auto vv) {
+    hipError_t e = with_row_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto vv) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &n,
-            reinterpret_cast<const void*>(
-                &line_march_kernel<decltype(kk)::value, decltype(dd)::value, decltype(vv)::value>),
+            reinterpret_cast<const void*>(&row_pass_kernel<decltype(ti), decltype(to), decltype(ac),
+                                                           decltype(kk)::value, decltype(vv)::value>),
             256, 0);
     });
     return e == hipSuccess ? n : 0;
 }
 
-bool row_pass_supported(int dt, int k, int v)
-{
-    return with_row_instance(dt, k, v, [](auto, auto) { return hipSuccess; }) == hipSuccess;
-}
-
-int row_pass_blocks_per_cu(int dt, int k, int v)
-{
-    int n = 0;
-    hipError_t e = with_row_instance(dt, k, v, [&](auto kk, auto vv) {
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, reinterpret_cast<const void*>(&row_pass_kernel<decltype(kk)::value, decltype(vv)::value>), 256, 0);
-    });
-    return e == hipSuccess ? n : 0;
-}
-
-hipError_t launch_row_pass(const RowArgs& a, int dt, int v, uint32_t blocks, hipStream_t s)
+hipError_t launch_row_pass(const RowArgs& a, const LineKey& key, uint32_t blocks, hipStream_t s)
 {
     if (a.total == 0 || blocks == 0) return hipSuccess;
-    return with_row_instance(dt, a.k, v, [&](auto kk, auto vv) {
-        hipLaunchKernelGGL((row_pass_kernel<decltype(kk)::value, decltype(vv)::value>), dim3(blocks), dim3(256), 0, s,
-                           a);
+    return with_row_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto vv) {
+        hipLaunchKernelGGL((row_pass_kernel<decltype(ti), decltype(to), decltype(ac), decltype(kk)::value,
+                                            decltype(vv)::value>),
+                           dim3(blocks), dim3(256), 0, s, a);
         return hipGetLastError();
     });
 }
 
-hipError_t launch_line_march(const LineArgs& a, int dt, int v, int d, hipStream_t s)
+bool line_march_supported(const LineKey& key)
+{
+    return with_line_instance(key, [](auto, auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
+}
+
+int line_march_blocks_per_cu(const LineKey& key)
+{
+    int n = 0;
+    hipError_t e = with_line_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto dd, auto vv) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n,
+            reinterpret_cast<const void*>(&line_march_kernel<decltype(ti), decltype(to), decltype(ac),
+                                                             decltype(kk)::value, decltype(dd)::value,
+                                                             decltype(vv)::value>),
+            256, 0);
+    });
+    return e == hipSuccess ? n : 0;
+}
+
+hipError_t launch_line_march(const LineArgs& a, const LineKey& key, hipStream_t s)
 {
     const uint64_t blocks = (uint64_t)a.tiles_z * ((a.lanes + 255u) / 256u);
     if (blocks == 0) return hipSuccess;
-    return with_line_instance(dt, a.k, v, d, [&](auto kk, auto dd, auto vv) {
-        hipLaunchKernelGGL((line_march_kernel<decltype(kk)::value, decltype(dd)::value, decltype(vv)::value>),
+    return with_line_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto dd, auto vv) {
+        hipLaunchKernelGGL((line_march_kernel<decltype(ti), decltype(to), decltype(ac), decltype(kk)::value,
+                                              decltype(dd)::value, decltype(vv)::value>),
                            dim3((uint32_t)blocks), dim3(256), 0, s, a);
         return hipGetLastError();
     });

@@ -245,12 +245,19 @@ struct LineArgs {
     FastDiv bext[NDL_MAX_DIM];
     int64_t sb[NDL_MAX_DIM];
     int64_t db[NDL_MAX_DIM];
-    float w[NDL_MAX_TAPS];
+    float w[NDL_MAX_TAPS];   // NDL_ACCUM_F32
+    double wd[NDL_MAX_TAPS]; // NDL_ACCUM_REF
 };
-// v: x elements per lane (1, 2, 4); d: loads in flight per lane
-hipError_t launch_line_march(const LineArgs& a, int dt, int v, int d, hipStream_t s);
-bool line_march_supported(int dt, int k, int v, int d);
-int line_march_blocks_per_cu(int dt, int k, int v, int d);
+// One-axis kernel selection: storage types (NDL_F32 / NDL_F64 in and out),
+// accumulation (NDL_ACCUM_F32: float in and out; NDL_ACCUM_REF: FP64, every
+// product and sum rounded separately), tap count, x elements per lane v
+// (1, 2, 4) and, for the line march, loads in flight per lane d.
+struct LineKey {
+    int in_dt, out_dt, accum, k, v, d;
+};
+hipError_t launch_line_march(const LineArgs& a, const LineKey& key, hipStream_t s);
+bool line_march_supported(const LineKey& key);
+int line_march_blocks_per_cu(const LineKey& key);
 // One-axis pass along the unit-stride dimension (kernels_line.hip): a wave
 // takes a segment of 64 x V outputs of one row (a fixed index of every other
 // dimension) and its halo through a wave-private LDS window.
@@ -265,11 +272,13 @@ struct RowArgs {
     FastDiv bext[NDL_MAX_DIM];
     int64_t sb[NDL_MAX_DIM];
     int64_t db[NDL_MAX_DIM];
-    float w[NDL_MAX_TAPS];
+    float w[NDL_MAX_TAPS];   // NDL_ACCUM_F32
+    double wd[NDL_MAX_TAPS]; // NDL_ACCUM_REF
 };
-hipError_t launch_row_pass(const RowArgs& a, int dt, int v, uint32_t blocks, hipStream_t s);
-bool row_pass_supported(int dt, int k, int v);
-int row_pass_blocks_per_cu(int dt, int k, int v);
+// key.d is unused for rows
+hipError_t launch_row_pass(const RowArgs& a, const LineKey& key, uint32_t blocks, hipStream_t s);
+bool row_pass_supported(const LineKey& key);
+int row_pass_blocks_per_cu(const LineKey& key);
 // resident workgroups per CU for this instantiation (0 if unknown)
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
 // whether an instantiation exists (queue: the persistent work-queue kernel)

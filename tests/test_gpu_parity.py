@@ -543,6 +543,51 @@ def test_row_pass_x_axis(case, border, monkeypatch):
     assert rel_err(tile, want) <= 1e-6
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", [("line", i) for i in range(len(LINE_CASES))] +
+                         [("row", i) for i in range(len(ROW_CASES))], ids=lambda c: f"{c[0]}{c[1]}")
+def test_one_axis_kernels_reference_bit_exact(case, dtype, monkeypatch):
+    # the reference's FP64 arithmetic (NDL_ACCUM_REF) on the line march and
+    # the row kernel: bit-identical to the oracle for float and double data
+    kind, i = case
+    if kind == "line":
+        shape, axis, k, view, env = LINE_CASES[i]
+        centre = None
+    else:
+        shape, k, centre, view, env = ROW_CASES[i]
+        axis = 0
+    border = ["clamp", "wrap", "reflect"][i % 3]
+    say("one axis REF", kind, shape, axis, k, border, dtype.__name__, env)
+    rng = np.random.default_rng(300 + i)
+    base = rng.random(shape).astype(dtype)
+    v = base[view] if view is not None else base
+    taps = [rng.random(k) + 0.05]
+    centers = None if centre is None else [centre]
+    want = oracle.sepconv(np.ascontiguousarray(v), [axis], taps, [border], centers=centers)
+    for name, val in env.items():
+        monkeypatch.setenv(name, val)
+    got = gpu_sep(base, [axis], taps, [border], REF, centers=centers, view=view)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_reference_chain_through_one_axis_kernels(dtype, monkeypatch):
+    # gaussian_blur's reference mode: three passes through FP64
+    # intermediates (row, then line march twice), bit-identical to the oracle
+    # and to the per-axis pass kernel
+    rng = np.random.default_rng(17)
+    x = rng.random((40, 37, 300)).astype(dtype)
+    taps = [workloads.gaussian_taps(s) for s in (1.5, 1.1, 2.2)]
+    borders = ["reflect", "clamp", "wrap"]
+    want = oracle.sepconv(x, [0, 1, 2], taps, borders)
+    say("REF chain", dtype.__name__)
+    got = gpu_sep(x, [0, 1, 2], taps, borders, REF)
+    np.testing.assert_array_equal(got, want)
+    monkeypatch.setenv("NDL_LINE_MARCH", "0")
+    monkeypatch.setenv("NDL_ROW_PASS", "0")
+    np.testing.assert_array_equal(gpu_sep(x, [0, 1, 2], taps, borders, REF), want)
+
+
 # ------------------------------------------------------- half-precision storage
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
 @pytest.mark.parametrize("border", ["clamp", "reflect"])
