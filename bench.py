@@ -300,19 +300,30 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
                   "GBps_alg": round(2 * x.size / ms / 1e6, 1), "bit_exact": bool(ok)})
         # the 256^3 Gaussian's axes one at a time (convolve() with a 1-D
         # kernel image per axis), cold cache: x on the row kernel, y and z
-        # on the line march
+        # on the line march; FP32 accumulation, then the reference's FP64
+        # arithmetic (bit-identical), then gaussian_blur()'s reference mode
+        # (three passes through FP64 intermediates)
         x, axes, taps, borders = workloads.config2a()
         s = ndl_amd.DeviceArray.from_numpy(x)
         d = ndl_amd.DeviceArray(x.shape, np.float32)
-        for a, name in zip(axes, "xyz"):
-            f = fwd_fn(s, d, [a], [taps[a]], [borders[a]], abi.ACCUM_F32)
-            f()
-            ms = _time_launches(hip, stream, f, reps, flush)
-            err = float(np.max(np.abs(d.to_numpy().astype(np.float64) -
-                                      oracle.sepconv(x, [a], [taps[a]], [borders[a]]).astype(np.float64))))
-            emit({"config": f"2a one axis: 256^3 f32, {len(taps[a])} taps on {name} only, reflect, cold",
-                  "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
-                  "GBps_alg": round(8 * x.size / ms / 1e6, 1), "max_abs_err": err})
+        for accum, mode in ((abi.ACCUM_F32, "fp32"), (abi.ACCUM_REF, "reference fp64")):
+            for a, name in zip(axes, "xyz"):
+                f = fwd_fn(s, d, [a], [taps[a]], [borders[a]], accum)
+                f()
+                ms = _time_launches(hip, stream, f, reps, flush)
+                want = oracle.sepconv(x, [a], [taps[a]], [borders[a]])
+                got = d.to_numpy()
+                emit({"config": f"2a one axis: 256^3 f32, {len(taps[a])} taps on {name} only, reflect, {mode}, cold",
+                      "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
+                      "GBps_alg": round(8 * x.size / ms / 1e6, 1),
+                      "max_abs_err": float(np.max(np.abs(got.astype(np.float64) - want.astype(np.float64)))),
+                      "bit_exact": bool(np.array_equal(got, want))})
+        f = fwd_fn(s, d, axes, taps, borders, abi.ACCUM_REF)
+        f()
+        ms = _time_launches(hip, stream, f, reps, flush)
+        emit({"config": "2a reference arithmetic: 256^3 f32 Gaussian 11/11/19 reflect, FP64 chain, cold",
+              "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
+              "bit_exact": bool(np.array_equal(d.to_numpy(), oracle.sepconv(x, axes, taps, borders)))})
     if 3 in configs:
         x, axes, taps, borders = workloads.config3()
         s = ndl_amd.DeviceArray.from_numpy(x)
