@@ -533,16 +533,21 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
     return true;
 }
 
-// One-axis kernels (kernels_line.hip): float32 / float64 storage, FP32
-// accumulation (float in and out) or the reference's FP64 arithmetic, odd
+// One-axis kernels (kernels_line.hip): FP32 accumulation over float32,
+// bf16 or f16 storage (the same type in and out), or the reference's FP64
+// arithmetic over float32 / float64 storage; odd
 // tap counts up to 25 and no zero tap (the reference skips those,
 // detail.h:164-174; the per-axis pass handles them).
 bool one_axis_types(const ndl_tensor* in, const ndl_tensor* out, const ndl_axis_kernel& ak, int accum,
                     LineKey& key)
 {
     const int it = in->dtype, ot = out->dtype;
-    if ((it != NDL_F32 && it != NDL_F64) || (ot != NDL_F32 && ot != NDL_F64)) return false;
-    if (accum == NDL_ACCUM_F32 ? (it != NDL_F32 || ot != NDL_F32) : accum != NDL_ACCUM_REF) return false;
+    if (accum == NDL_ACCUM_F32) {
+        // float, bf16 or f16 storage, the same type in and out
+        if (it != ot || (it != NDL_F32 && it != NDL_BF16 && it != NDL_F16)) return false;
+    } else if (accum != NDL_ACCUM_REF || (it != NDL_F32 && it != NDL_F64) || (ot != NDL_F32 && ot != NDL_F64)) {
+        return false;
+    }
     for (int t = 0; t < ak.count; t++)
         if (ak.taps[t] == 0.0) return false;
     key = LineKey{it, ot, accum, ak.count, 1, 8};

@@ -1,4 +1,5 @@
-// kernels_line.hip -- one-axis passes over float32 / float64 storage: a 1-D
+// kernels_line.hip -- one-axis passes over float32 / float64 storage (and
+// bf16 / f16 storage with FP32 accumulation): a 1-D
 // kernel image {k, 1, 1}, {1, k, 1}, ... given to convolve()
 // (processing/convolution.h:52-71), or one per-axis pass of gaussian_blur()
 // (convolution.h:107-186, through FP64 intermediates).
@@ -54,6 +55,18 @@ template <> __device__ __forceinline__ double mac1<double>(double acc, double w,
 }
 template <> __device__ __forceinline__ float mac1<float>(float acc, float w, float x) { return fmaf(w, x, acc); }
 
+// storage element <-> accumulator (bf16 / f16 through their bit patterns)
+template <class Acc, class T> __device__ __forceinline__ Acc to_acc(const T& x)
+{
+    if constexpr (sizeof(Acc) == 8) return load_f64(&x);
+    else return load_f32(&x);
+}
+template <class Acc, class T> __device__ __forceinline__ void from_acc(T& o, Acc v)
+{
+    if constexpr (sizeof(Acc) == 8) store_from_f64(&o, v, NDL_ROUND_TRUNC);
+    else store_from_f32(&o, v);
+}
+
 template <class Acc> __device__ __forceinline__ Acc tap(const float* w, const double* wd, int t);
 template <> __device__ __forceinline__ double tap<double>(const float*, const double* wd, int t) { return wd[t]; }
 template <> __device__ __forceinline__ float tap<float>(const float* w, const double*, int t) { return w[t]; }
@@ -108,11 +121,11 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
                 for (int t = 0; t < K; t++)
 #pragma unroll
                     for (int e = 0; e < V; e++)
-                        acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), (Acc)ring[(u - (K - 1) + t + R) % R][e]);
+                        acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), to_acc<Acc>(ring[(u - (K - 1) + t + R) % R][e]));
                 if (live) {
                     TOut o[V];
 #pragma unroll
-                    for (int e = 0; e < V; e++) o[e] = (TOut)acc[e];
+                    for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e]);
                     store_v<TOut, V>(dcol + (int64_t)(z0 + i - (K - 1)) * a.dz, o);
                 }
             }
@@ -180,7 +193,7 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
     };
 
     uint32_t seg = blockIdx.x * 4u + (uint32_t)wv;
-    TIn m[V], h = 0;
+    TIn m[V], h{};
     if (seg < a.total) issue(seg, m, h);
     for (; seg < a.total; seg += nwaves) {
         wave_lds_sync();  // the previous segment's window reads are done
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
 #pragma unroll
         for (int t = 0; t < K; t++)
 #pragma unroll
-            for (int e = 0; e < V; e++) acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), (Acc)wv_[e + t]);
+            for (int e = 0; e < V; e++) acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), to_acc<Acc>(wv_[e + t]));
         int64_t so, dof;
         int x0;
         row_of(seg, so, dof, x0);
@@ -205,7 +218,7 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
         const int xo = x0 + V * lane;
         TOut o[V];
 #pragma unroll
-        for (int e = 0; e < V; e++) o[e] = (TOut)acc[e];
+        for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e]);
         if (xo + V - 1 < a.X) {
             store_v<TOut, V>(drow + xo, o);
         } else {
@@ -221,6 +234,8 @@ template <class F> hipError_t with_types(const LineKey& key, F&& f)
 {
     if (key.accum == NDL_ACCUM_F32) {
         if (key.in_dt == NDL_F32 && key.out_dt == NDL_F32) return f(float{}, float{}, float{});
+        if (key.in_dt == NDL_BF16 && key.out_dt == NDL_BF16) return f(bf16_t{}, bf16_t{}, float{});
+        if (key.in_dt == NDL_F16 && key.out_dt == NDL_F16) return f(f16_t{}, f16_t{}, float{});
         return hipErrorInvalidValue;
     }
     if (key.accum != NDL_ACCUM_REF) return hipErrorInvalidValue;
@@ -245,12 +260,17 @@ template <class F> hipError_t with_k(int k, F&& f)
 // f(TIn, TOut, Acc, K, D, V): line march instances.  Float input: V 4, 2, 1
 // with 8 loads in flight (and V 4 with 4 for FP32 accumulation; no V 4 with
 // FP64 accumulators beyond 21 taps, whose ring and sums would spill); double
-// input: V 2, 1 (16 bytes per lane at most).
+// input: V 2, 1 (16 bytes per lane at most); bf16 / f16: V 4, 1.
 template <class F> hipError_t with_line_instance(const LineKey& key, F&& f)
 {
     return with_types(key, [&](auto ti, auto to, auto ac) {
         using TIn = decltype(ti);
         return with_k(key.k, [&](auto kk) {
+            if constexpr (sizeof(TIn) == 2) {
+                if (key.v == 4 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<4>{});
+                if (key.v == 1 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<1>{});
+                return hipErrorInvalidValue;
+            }
             if constexpr (sizeof(TIn) == 4 && (sizeof(decltype(ac)) == 4 || decltype(kk)::value <= 21)) {
                 if (key.v == 4 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<4>{});
                 if constexpr (std::is_same_v<decltype(ac), float>)

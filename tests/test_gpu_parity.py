@@ -570,6 +570,57 @@ def test_one_axis_kernels_reference_bit_exact(case, dtype, monkeypatch):
     np.testing.assert_array_equal(got, want)
 
 
+def _half(v, storage):
+    """(storage array, float64 of its values): f16, or bf16 bits rounded to nearest even."""
+    if storage == "f16":
+        xs = v.astype(np.float16)
+        return xs, xs.astype(np.float64)
+    u = v.astype(np.float32).view(np.uint32).astype(np.uint64)
+    xs = ((u + np.uint64(0x7FFF) + ((u >> np.uint64(16)) & np.uint64(1))) >> np.uint64(16)).astype(np.uint16)
+    return xs, oracle.bf16_to_f64(xs)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("case", [("line", i) for i in (0, 1, 4, 6, 8, 9)] + [("row", i) for i in (0, 1, 3, 6, 7, 8)],
+                         ids=lambda c: f"{c[0]}{c[1]}")
+def test_one_axis_kernels_half_storage(case, storage, monkeypatch):
+    # bf16 / f16 storage with FP32 accumulation on the line march and the row
+    # kernel; the FP64 oracle is fed the storage-rounded input, so the error
+    # is accumulation + one output rounding (2^-8 relative for bf16, 2^-11
+    # for f16); the tile / per-axis path is held to the same bar
+    kind, i = case
+    if kind == "line":
+        shape, axis, k, view, env = LINE_CASES[i]
+        centre = None
+    else:
+        shape, k, centre, view, env = ROW_CASES[i]
+        axis = 0
+    border = ["clamp", "wrap", "reflect"][i % 3]
+    say("one axis half", storage, kind, shape, axis, k, border)
+    rng = np.random.default_rng(400 + i)
+    xs, xd = _half(rng.random(shape) * 8 - 4, storage)
+    bf = storage == "bf16"
+    taps = [rng.random(k) + 0.05]
+    centers = None if centre is None else [centre]
+    vd = xd[view] if view is not None else xd
+    want = oracle.sepconv(np.ascontiguousarray(vd), [axis], taps, [border], dst=np.zeros(vd.shape), centers=centers)
+    ulp = 2.0 ** -8 if bf else 2.0 ** -11
+    scale = np.abs(xd).max() * np.abs(taps[0]).sum()
+    for name, val in env.items():
+        monkeypatch.setenv(name, val)
+    for off in (False, True):
+        if off:
+            monkeypatch.setenv("NDL_LINE_MARCH", "0")
+            monkeypatch.setenv("NDL_ROW_PASS", "0")
+        d = ndl_amd.DeviceArray.from_numpy(xs, bf16=bf)
+        src = d.view(view) if view is not None else d
+        out = ndl_amd.DeviceArray(src.shape, xs.dtype, bf16=bf)
+        ndl_amd.sepconv_dev(src, out, [axis], taps, [border], F32, centers=centers)
+        got = out.to_numpy()
+        got = oracle.bf16_to_f64(got) if bf else got.astype(np.float64)
+        assert np.all(np.abs(got - want) <= ulp * np.abs(want) + 1e-6 * scale), (off, np.abs(got - want).max())
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_reference_chain_through_one_axis_kernels(dtype, monkeypatch):
     # gaussian_blur's reference mode: three passes through FP64
