@@ -585,9 +585,12 @@ bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_ax
     // addresses are aligned and that have an instance
     int v = std::min(16 / (int)dtype_size(in->dtype), env_int("NDL_LINE_V", 4));
     v = v >= 4 ? 4 : (v >= 2 ? 2 : 1);
-    key.d = env_int("NDL_LINE_D", 8);
+    // loads in flight: 12 for float lanes of four with FP32 sums (measured
+    // 3-6% faster than 8 on 256^3), 8 otherwise; NDL_LINE_D overrides
+    const int d_env = env_int("NDL_LINE_D", 0);
     for (;; v /= 2) {
         key.v = v;
+        key.d = d_env > 0 ? d_env : (v == 4 && accum == NDL_ACCUM_F32 && in->dtype == NDL_F32 ? 12 : 8);
         if (vec_fits(in, out, xd, v) && line_march_supported(key)) break;
         if (v == 1) return false;
     }

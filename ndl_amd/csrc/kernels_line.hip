@@ -258,7 +258,7 @@ template <class F> hipError_t with_k(int k, F&& f)
 }
 
 // f(TIn, TOut, Acc, K, D, V): line march instances.  Float input: V 4, 2, 1
-// with 8 loads in flight (and V 4 with 4 for FP32 accumulation; no V 4 with
+// with 8 loads in flight (and V 4 with 4 or 12 for FP32 accumulation; no V 4 with
 // FP64 accumulators beyond 21 taps, whose ring and sums would spill); double
 // input: V 2, 1 (16 bytes per lane at most); bf16 / f16: V 4, 1.
 template <class F> hipError_t with_line_instance(const LineKey& key, F&& f)
@@ -273,8 +273,10 @@ template <class F> hipError_t with_line_instance(const LineKey& key, F&& f)
             }
             if constexpr (sizeof(TIn) == 4 && (sizeof(decltype(ac)) == 4 || decltype(kk)::value <= 21)) {
                 if (key.v == 4 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<4>{});
-                if constexpr (std::is_same_v<decltype(ac), float>)
+                if constexpr (std::is_same_v<decltype(ac), float>) {
                     if (key.v == 4 && key.d == 4) return f(ti, to, ac, kk, IC<4>{}, IC<4>{});
+                    if (key.v == 4 && key.d == 12) return f(ti, to, ac, kk, IC<12>{}, IC<4>{});
+                }
             }
             if (key.v == 2 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<2>{});
             if (key.v == 1 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<1>{});

@@ -482,6 +482,7 @@ LINE_CASES = [
     ((101, 9, 64), 2, 19, None, {"NDL_LINE_V": "2"}),       # two elements per lane
     ((101, 9, 64), 2, 13, None, {"NDL_LINE_D": "4"}),       # four per lane, shallow prefetch
     ((64, 40, 256), 2, 11, None, {"NDL_LINE_TZ": "8"}),     # many chunks of many lane blocks
+    ((101, 9, 64), 2, 17, None, {"NDL_LINE_D": "8"}),       # four per lane, 8 in flight
 ]
 
 
