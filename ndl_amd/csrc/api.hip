@@ -210,7 +210,7 @@ int run_axis_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const n
 }
 
 int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
-                 hipStream_t s);
+                 hipStream_t s, int op = 0);
 
 // per-axis chain through dense accumulator-typed intermediates
 int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
@@ -539,10 +539,14 @@ bool plan_fused(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, con
 // tap counts up to 25 and no zero tap (the reference skips those,
 // detail.h:164-174; the per-axis pass handles them).
 bool one_axis_types(const ndl_tensor* in, const ndl_tensor* out, const ndl_axis_kernel& ak, int accum,
-                    LineKey& key)
+                    LineKey& key, int op)
 {
     const int it = in->dtype, ot = out->dtype;
-    if (accum == NDL_ACCUM_F32) {
+    if (numel(in) == 0) return false;  // no extent may be 0 below (chunking, divisors)
+    if (op != 0) {
+        // erode / dilate over a box (every tap included), float32 or uint8
+        if (it != ot || (it != NDL_F32 && it != NDL_U8)) return false;
+    } else if (accum == NDL_ACCUM_F32) {
         // float, bf16 or f16 storage, the same type in and out
         if (it != ot || (it != NDL_F32 && it != NDL_BF16 && it != NDL_F16)) return false;
     } else if (accum != NDL_ACCUM_REF || (it != NDL_F32 && it != NDL_F64) || (ot != NDL_F32 && ot != NDL_F64)) {
@@ -550,7 +554,7 @@ bool one_axis_types(const ndl_tensor* in, const ndl_tensor* out, const ndl_axis_
     }
     for (int t = 0; t < ak.count; t++)
         if (ak.taps[t] == 0.0) return false;
-    key = LineKey{it, ot, accum, ak.count, 1, 8};
+    key = LineKey{it, ot, accum, ak.count, 1, 8, op};
     return true;
 }
 
@@ -576,9 +580,9 @@ bool vec_fits(const ndl_tensor* in, const ndl_tensor* out, int xd, int v)
 
 // An axis other than the unit-stride one: the line march.
 bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_axis_kernel& ak, int accum,
-               LineArgs& la, LineKey& key)
+               LineArgs& la, LineKey& key, int op)
 {
-    if (!env_int("NDL_LINE_MARCH", 1) || !one_axis_types(in, out, ak, accum, key)) return false;
+    if (!env_int("NDL_LINE_MARCH", 1) || !one_axis_types(in, out, ak, accum, key, op)) return false;
     const int nd = in->ndim, xd = unit_dim(in, out);
     if (xd < 0 || xd == zd || in->extent[xd] < 64 || in->extent[zd] > 0x7fffffff) return false;
     // the widest lanes (16 bytes at most, NDL_LINE_V caps them) whose
@@ -590,7 +594,7 @@ bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_ax
     const int d_env = env_int("NDL_LINE_D", 0);
     for (;; v /= 2) {
         key.v = v;
-        key.d = d_env > 0 ? d_env : (v == 4 && accum == NDL_ACCUM_F32 && in->dtype == NDL_F32 ? 12 : 8);
+        key.d = d_env > 0 ? d_env : (v == 4 && op == 0 && accum == NDL_ACCUM_F32 && in->dtype == NDL_F32 ? 12 : 8);
         if (vec_fits(in, out, xd, v) && line_march_supported(key)) break;
         if (v == 1) return false;
     }
@@ -641,9 +645,9 @@ bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_ax
 
 // The unit-stride axis: the row kernel.  blocks = grid size.
 bool plan_row(const ndl_tensor* in, const ndl_tensor* out, int xd, const ndl_axis_kernel& ak, int accum,
-              RowArgs& ra, LineKey& key, uint32_t& blocks)
+              RowArgs& ra, LineKey& key, uint32_t& blocks, int op)
 {
-    if (!env_int("NDL_ROW_PASS", 1) || !one_axis_types(in, out, ak, accum, key)) return false;
+    if (!env_int("NDL_ROW_PASS", 1) || !one_axis_types(in, out, ak, accum, key, op)) return false;
     const int nd = in->ndim;
     if (in->stride[xd] != 1 || out->stride[xd] != 1 || in->extent[xd] < 32 || in->extent[xd] > 0x7fffffff)
         return false;
@@ -680,19 +684,20 @@ bool plan_row(const ndl_tensor* in, const ndl_tensor* out, int xd, const ndl_axi
 }
 
 // One axis through the line march or the row kernel: NDL_OK when it ran,
-// -1 when neither applies, an error code otherwise.
+// -1 when neither applies, an error code otherwise.  op: 0 convolution,
+// 1 erode, 2 dilate.
 int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
-                 hipStream_t s)
+                 hipStream_t s, int op)
 {
     LineKey key;
     LineArgs la;
-    if (plan_line(in, out, axis, ak, accum, la, key)) {
+    if (plan_line(in, out, axis, ak, accum, la, key, op)) {
         hipError_t e = launch_line_march(la, key, s);
         return e == hipSuccess ? NDL_OK : hip_fail(e, "line march launch");
     }
     RowArgs ra;
     uint32_t blocks = 0;
-    if (plan_row(in, out, axis, ak, accum, ra, key, blocks)) {
+    if (plan_row(in, out, axis, ak, accum, ra, key, blocks, op)) {
         hipError_t e = launch_row_pass(ra, key, blocks, s);
         return e == hipSuccess ? NDL_OK : hip_fail(e, "row pass launch");
     }
@@ -1153,6 +1158,11 @@ int run_morph_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const 
 {
     int64_t n = numel(in);
     if (n == 0) return NDL_OK;
+    // a box pass without the corner rule: the one-axis kernels
+    if (!corner) {
+        const int rc = try_one_axis(in, out, axis, k, NDL_ACCUM_F32, s, dilate ? 2 : 1);
+        if (rc != -1) return rc;
+    }
     MorphPassArgs a;
     memset(&a, 0, sizeof a);
     a.in = in->data;

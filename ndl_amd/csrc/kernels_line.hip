@@ -67,11 +67,22 @@ template <class Acc, class T> __device__ __forceinline__ void from_acc(T& o, Acc
     else store_from_f32(&o, v);
 }
 
+// erode (OP 1) / dilate (OP 2): whether v replaces the current best; taps
+// run in ascending order, strict comparisons (the first tap wins ties, so a
+// -0 / +0 tie keeps the earlier one) and a NaN never wins over a number
+// (processing/morphology.h:57-101 with the NaN rule of the separable passes)
+template <int OP, class T> __device__ __forceinline__ bool morph_take(T best, T v)
+{
+    if constexpr (std::is_floating_point<T>::value)
+        if (best != best) return v == v;
+    return OP == 2 ? v > best : v < best;
+}
+
 template <class Acc> __device__ __forceinline__ Acc tap(const float* w, const double* wd, int t);
 template <> __device__ __forceinline__ double tap<double>(const float*, const double* wd, int t) { return wd[t]; }
 template <> __device__ __forceinline__ float tap<float>(const float* w, const double*, int t) { return w[t]; }
 
-template <class TIn, class TOut, class Acc, int K, int D, int V>
+template <class TIn, class TOut, class Acc, int OP, int K, int D, int V>
 __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
 {
     constexpr int R = K + D;  // ring: K inputs of the current output plus D in flight
@@ -114,20 +125,32 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
             const int i = i0 + u;
             if (i + D < n_in) load(i + D, ring[(u + D) % R]);
             if (i >= K - 1 && i < n_in) {
-                Acc acc[V];
+                TOut o[V];
+                if constexpr (OP == 0) {
+                    Acc acc[V];
 #pragma unroll
-                for (int e = 0; e < V; e++) acc[e] = 0;
+                    for (int e = 0; e < V; e++) acc[e] = 0;
 #pragma unroll
-                for (int t = 0; t < K; t++)
+                    for (int t = 0; t < K; t++)
 #pragma unroll
-                    for (int e = 0; e < V; e++)
-                        acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), to_acc<Acc>(ring[(u - (K - 1) + t + R) % R][e]));
-                if (live) {
-                    TOut o[V];
+                        for (int e = 0; e < V; e++)
+                            acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t),
+                                               to_acc<Acc>(ring[(u - (K - 1) + t + R) % R][e]));
 #pragma unroll
                     for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e]);
-                    store_v<TOut, V>(dcol + (int64_t)(z0 + i - (K - 1)) * a.dz, o);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; e++) {
+                        TIn best = ring[(u - (K - 1) + R) % R][e];
+#pragma unroll
+                        for (int t = 1; t < K; t++) {
+                            const TIn v = ring[(u - (K - 1) + t + R) % R][e];
+                            if (morph_take<OP>(best, v)) best = v;
+                        }
+                        o[e] = best;
+                    }
                 }
+                if (live) store_v<TOut, V>(dcol + (int64_t)(z0 + i - (K - 1)) * a.dz, o);
             }
         }
     }
@@ -147,7 +170,7 @@ __device__ __forceinline__ void wave_lds_sync()
 // LDS window; each lane then reads its K+V-1 window values and writes V
 // outputs.  The next segment's loads are issued before the current one is
 // computed.  Only wave-level ordering is needed (no workgroup barrier).
-template <class TIn, class TOut, class Acc, int K, int V>
+template <class TIn, class TOut, class Acc, int OP, int K, int V>
 __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
 {
     constexpr int L = 64 * V;
@@ -204,21 +227,33 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
         TIn wv_[K + V - 1];
 #pragma unroll
         for (int j = 0; j < K + V - 1; j++) wv_[j] = buf[hl - c + V * lane + j];
-        Acc acc[V];
+        TOut o[V];
+        if constexpr (OP == 0) {
+            Acc acc[V];
 #pragma unroll
-        for (int e = 0; e < V; e++) acc[e] = 0;
+            for (int e = 0; e < V; e++) acc[e] = 0;
 #pragma unroll
-        for (int t = 0; t < K; t++)
+            for (int t = 0; t < K; t++)
 #pragma unroll
-            for (int e = 0; e < V; e++) acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), to_acc<Acc>(wv_[e + t]));
+                for (int e = 0; e < V; e++)
+                    acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), to_acc<Acc>(wv_[e + t]));
+#pragma unroll
+            for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                TIn best = wv_[e];
+#pragma unroll
+                for (int t = 1; t < K; t++)
+                    if (morph_take<OP>(best, wv_[e + t])) best = wv_[e + t];
+                o[e] = best;
+            }
+        }
         int64_t so, dof;
         int x0;
         row_of(seg, so, dof, x0);
         TOut* drow = (TOut*)a.dst + dof;
         const int xo = x0 + V * lane;
-        TOut o[V];
-#pragma unroll
-        for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e]);
         if (xo + V - 1 < a.X) {
             store_v<TOut, V>(drow + xo, o);
         } else {
@@ -229,20 +264,29 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
     }
 }
 
-// f(TIn, TOut, Acc) for the storage / accumulation combinations
+// f(TIn, TOut, Acc, OP) for the storage / accumulation / operation
+// combinations (erode and dilate: float32 and uint8, the same type in and out)
 template <class F> hipError_t with_types(const LineKey& key, F&& f)
 {
+    if (key.op == 1 || key.op == 2) {
+        if (key.in_dt != key.out_dt) return hipErrorInvalidValue;
+        if (key.in_dt == NDL_F32) return key.op == 1 ? f(float{}, float{}, float{}, IC<1>{}) : f(float{}, float{}, float{}, IC<2>{});
+        if (key.in_dt == NDL_U8)
+            return key.op == 1 ? f(uint8_t{}, uint8_t{}, float{}, IC<1>{}) : f(uint8_t{}, uint8_t{}, float{}, IC<2>{});
+        return hipErrorInvalidValue;
+    }
+    if (key.op != 0) return hipErrorInvalidValue;
     if (key.accum == NDL_ACCUM_F32) {
-        if (key.in_dt == NDL_F32 && key.out_dt == NDL_F32) return f(float{}, float{}, float{});
-        if (key.in_dt == NDL_BF16 && key.out_dt == NDL_BF16) return f(bf16_t{}, bf16_t{}, float{});
-        if (key.in_dt == NDL_F16 && key.out_dt == NDL_F16) return f(f16_t{}, f16_t{}, float{});
+        if (key.in_dt == NDL_F32 && key.out_dt == NDL_F32) return f(float{}, float{}, float{}, IC<0>{});
+        if (key.in_dt == NDL_BF16 && key.out_dt == NDL_BF16) return f(bf16_t{}, bf16_t{}, float{}, IC<0>{});
+        if (key.in_dt == NDL_F16 && key.out_dt == NDL_F16) return f(f16_t{}, f16_t{}, float{}, IC<0>{});
         return hipErrorInvalidValue;
     }
     if (key.accum != NDL_ACCUM_REF) return hipErrorInvalidValue;
-    if (key.in_dt == NDL_F32 && key.out_dt == NDL_F32) return f(float{}, float{}, double{});
-    if (key.in_dt == NDL_F32 && key.out_dt == NDL_F64) return f(float{}, double{}, double{});
-    if (key.in_dt == NDL_F64 && key.out_dt == NDL_F64) return f(double{}, double{}, double{});
-    if (key.in_dt == NDL_F64 && key.out_dt == NDL_F32) return f(double{}, float{}, double{});
+    if (key.in_dt == NDL_F32 && key.out_dt == NDL_F32) return f(float{}, float{}, double{}, IC<0>{});
+    if (key.in_dt == NDL_F32 && key.out_dt == NDL_F64) return f(float{}, double{}, double{}, IC<0>{});
+    if (key.in_dt == NDL_F64 && key.out_dt == NDL_F64) return f(double{}, double{}, double{}, IC<0>{});
+    if (key.in_dt == NDL_F64 && key.out_dt == NDL_F32) return f(double{}, float{}, double{}, IC<0>{});
     return hipErrorInvalidValue;
 }
 
@@ -257,42 +301,54 @@ template <class F> hipError_t with_k(int k, F&& f)
     return hipErrorInvalidValue;
 }
 
-// f(TIn, TOut, Acc, K, D, V): line march instances.  Float input: V 4, 2, 1
-// with 8 loads in flight (and V 4 with 4 or 12 for FP32 accumulation; no V 4 with
-// FP64 accumulators beyond 21 taps, whose ring and sums would spill); double
-// input: V 2, 1 (16 bytes per lane at most); bf16 / f16: V 4, 1.
+// f(TIn, TOut, Acc, OP, K, D, V): line march instances.  Float input: V 4,
+// 2, 1 with 8 loads in flight (and V 4 with 4 or 12 for FP32 accumulation;
+// no V 4 with FP64 accumulators beyond 21 taps, whose ring and sums would
+// spill); double input: V 2, 1 (16 bytes per lane at most); bf16 / f16 and
+// uint8: V 4, 1.  Erode / dilate: box sizes up to 11.
 template <class F> hipError_t with_line_instance(const LineKey& key, F&& f)
 {
-    return with_types(key, [&](auto ti, auto to, auto ac) {
+    return with_types(key, [&](auto ti, auto to, auto ac, auto op) {
         using TIn = decltype(ti);
+        constexpr int OP = decltype(op)::value;
         return with_k(key.k, [&](auto kk) {
-            if constexpr (sizeof(TIn) == 2) {
-                if (key.v == 4 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<4>{});
-                if (key.v == 1 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<1>{});
+            if constexpr (OP != 0 && decltype(kk)::value > 11) {
+                return hipErrorInvalidValue;
+            } else if constexpr (sizeof(TIn) <= 2) {
+                if (key.v == 4 && key.d == 8) return f(ti, to, ac, op, kk, IC<8>{}, IC<4>{});
+                if (key.v == 1 && key.d == 8) return f(ti, to, ac, op, kk, IC<8>{}, IC<1>{});
+                return hipErrorInvalidValue;
+            } else {
+                if constexpr (sizeof(TIn) == 4 && (sizeof(decltype(ac)) == 4 || decltype(kk)::value <= 21)) {
+                    if (key.v == 4 && key.d == 8) return f(ti, to, ac, op, kk, IC<8>{}, IC<4>{});
+                    if constexpr (std::is_same_v<decltype(ac), float> && OP == 0) {
+                        if (key.v == 4 && key.d == 4) return f(ti, to, ac, op, kk, IC<4>{}, IC<4>{});
+                        if (key.v == 4 && key.d == 12) return f(ti, to, ac, op, kk, IC<12>{}, IC<4>{});
+                    }
+                }
+                if constexpr (OP == 0) {
+                    if (key.v == 2 && key.d == 8) return f(ti, to, ac, op, kk, IC<8>{}, IC<2>{});
+                }
+                if (key.v == 1 && key.d == 8) return f(ti, to, ac, op, kk, IC<8>{}, IC<1>{});
                 return hipErrorInvalidValue;
             }
-            if constexpr (sizeof(TIn) == 4 && (sizeof(decltype(ac)) == 4 || decltype(kk)::value <= 21)) {
-                if (key.v == 4 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<4>{});
-                if constexpr (std::is_same_v<decltype(ac), float>) {
-                    if (key.v == 4 && key.d == 4) return f(ti, to, ac, kk, IC<4>{}, IC<4>{});
-                    if (key.v == 4 && key.d == 12) return f(ti, to, ac, kk, IC<12>{}, IC<4>{});
-                }
-            }
-            if (key.v == 2 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<2>{});
-            if (key.v == 1 && key.d == 8) return f(ti, to, ac, kk, IC<8>{}, IC<1>{});
-            return hipErrorInvalidValue;
         });
     });
 }
 
-// f(TIn, TOut, Acc, K, V): row instances, V 4 or 1
+// f(TIn, TOut, Acc, OP, K, V): row instances, V 4 or 1 (erode / dilate: box
+// sizes up to 11)
 template <class F> hipError_t with_row_instance(const LineKey& key, F&& f)
 {
-    return with_types(key, [&](auto ti, auto to, auto ac) {
+    return with_types(key, [&](auto ti, auto to, auto ac, auto op) {
         return with_k(key.k, [&](auto kk) {
-            if (key.v == 4) return f(ti, to, ac, kk, IC<4>{});
-            if (key.v == 1) return f(ti, to, ac, kk, IC<1>{});
-            return hipErrorInvalidValue;
+            if constexpr (decltype(op)::value != 0 && decltype(kk)::value > 11) {
+                return hipErrorInvalidValue;
+            } else {
+                if (key.v == 4) return f(ti, to, ac, op, kk, IC<4>{});
+                if (key.v == 1) return f(ti, to, ac, op, kk, IC<1>{});
+                return hipErrorInvalidValue;
+            }
         });
     });
 }
@@ -301,17 +357,18 @@ template <class F> hipError_t with_row_instance(const LineKey& key, F&& f)
 
 bool row_pass_supported(const LineKey& key)
 {
-    return with_row_instance(key, [](auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
+    return with_row_instance(key, [](auto, auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
 }
 
 int row_pass_blocks_per_cu(const LineKey& key)
 {
     int n = 0;
-    hipError_t e = with_row_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto vv) {
+    hipError_t e = with_row_instance(key, [&](auto ti, auto to, auto ac, auto op, auto kk, auto vv) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &n,
             reinterpret_cast<const void*>(&row_pass_kernel<decltype(ti), decltype(to), decltype(ac),
-                                                           decltype(kk)::value, decltype(vv)::value>),
+                                                           decltype(op)::value, decltype(kk)::value,
+                                                           decltype(vv)::value>),
             256, 0);
     });
     return e == hipSuccess ? n : 0;
@@ -320,9 +377,9 @@ int row_pass_blocks_per_cu(const LineKey& key)
 hipError_t launch_row_pass(const RowArgs& a, const LineKey& key, uint32_t blocks, hipStream_t s)
 {
     if (a.total == 0 || blocks == 0) return hipSuccess;
-    return with_row_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto vv) {
-        hipLaunchKernelGGL((row_pass_kernel<decltype(ti), decltype(to), decltype(ac), decltype(kk)::value,
-                                            decltype(vv)::value>),
+    return with_row_instance(key, [&](auto ti, auto to, auto ac, auto op, auto kk, auto vv) {
+        hipLaunchKernelGGL((row_pass_kernel<decltype(ti), decltype(to), decltype(ac), decltype(op)::value,
+                                            decltype(kk)::value, decltype(vv)::value>),
                            dim3(blocks), dim3(256), 0, s, a);
         return hipGetLastError();
     });
@@ -330,18 +387,19 @@ hipError_t launch_row_pass(const RowArgs& a, const LineKey& key, uint32_t blocks
 
 bool line_march_supported(const LineKey& key)
 {
-    return with_line_instance(key, [](auto, auto, auto, auto, auto, auto) { return hipSuccess; }) == hipSuccess;
+    return with_line_instance(key, [](auto, auto, auto, auto, auto, auto, auto) { return hipSuccess; }) ==
+           hipSuccess;
 }
 
 int line_march_blocks_per_cu(const LineKey& key)
 {
     int n = 0;
-    hipError_t e = with_line_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto dd, auto vv) {
+    hipError_t e = with_line_instance(key, [&](auto ti, auto to, auto ac, auto op, auto kk, auto dd, auto vv) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &n,
             reinterpret_cast<const void*>(&line_march_kernel<decltype(ti), decltype(to), decltype(ac),
-                                                             decltype(kk)::value, decltype(dd)::value,
-                                                             decltype(vv)::value>),
+                                                             decltype(op)::value, decltype(kk)::value,
+                                                             decltype(dd)::value, decltype(vv)::value>),
             256, 0);
     });
     return e == hipSuccess ? n : 0;
@@ -351,9 +409,9 @@ hipError_t launch_line_march(const LineArgs& a, const LineKey& key, hipStream_t 
 {
     const uint64_t blocks = (uint64_t)a.tiles_z * ((a.lanes + 255u) / 256u);
     if (blocks == 0) return hipSuccess;
-    return with_line_instance(key, [&](auto ti, auto to, auto ac, auto kk, auto dd, auto vv) {
-        hipLaunchKernelGGL((line_march_kernel<decltype(ti), decltype(to), decltype(ac), decltype(kk)::value,
-                                              decltype(dd)::value, decltype(vv)::value>),
+    return with_line_instance(key, [&](auto ti, auto to, auto ac, auto op, auto kk, auto dd, auto vv) {
+        hipLaunchKernelGGL((line_march_kernel<decltype(ti), decltype(to), decltype(ac), decltype(op)::value,
+                                              decltype(kk)::value, decltype(dd)::value, decltype(vv)::value>),
                            dim3((uint32_t)blocks), dim3(256), 0, s, a);
         return hipGetLastError();
     });

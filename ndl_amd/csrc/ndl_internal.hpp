@@ -248,12 +248,14 @@ struct LineArgs {
     float w[NDL_MAX_TAPS];   // NDL_ACCUM_F32
     double wd[NDL_MAX_TAPS]; // NDL_ACCUM_REF
 };
-// One-axis kernel selection: storage types (NDL_F32 / NDL_F64 in and out),
+// One-axis kernel selection (erode / dilate: float32 or uint8, the same type
+// in and out, every tap included): storage types (NDL_F32 / NDL_F64 in and out),
 // accumulation (NDL_ACCUM_F32: float in and out; NDL_ACCUM_REF: FP64, every
 // product and sum rounded separately), tap count, x elements per lane v
 // (1, 2, 4) and, for the line march, loads in flight per lane d.
 struct LineKey {
     int in_dt, out_dt, accum, k, v, d;
+    int op;  // 0 convolution, 1 erode (minimum), 2 dilate (maximum)
 };
 hipError_t launch_line_march(const LineArgs& a, const LineKey& key, hipStream_t s);
 bool line_march_supported(const LineKey& key);

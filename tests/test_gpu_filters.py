@@ -216,3 +216,30 @@ def test_scale_stack_uint8_and_fp32():
     for lv, sg in zip(levels, sig):
         want = oracle.gaussian_blur(xf.astype(np.float64), sg, "reflect")
         assert np.max(np.abs(lv.to_numpy() - want) / np.maximum(np.abs(want), 1e-30)) <= 1e-6
+
+
+MORPH_ONE_AXIS = [
+    # (dtype, shape, box): the passes before the last (every pass for integer
+    # images) run on the row kernel / line march
+    (np.float32, (30, 20, 260), (5, 3, 7)),
+    (np.float32, (300, 260), (5, 9)),
+    (np.uint8, (3, 300, 700), (1, 7, 7)),
+    (np.uint8, (40, 33, 96), (3, 11, 5)),
+]
+
+
+@pytest.mark.parametrize("op", ["erode", "dilate"])
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+@pytest.mark.parametrize("case", range(len(MORPH_ONE_AXIS)))
+def test_morph_box_one_axis_kernels(case, border, op, monkeypatch):
+    dtype, shape, box = MORPH_ONE_AXIS[case]
+    rng = np.random.default_rng(500 + case)
+    x = _data(dtype, shape, rng)
+    mask = np.ones(box)
+    say("morph one axis", dtype.__name__, shape, box, border, op)
+    want = oracle.morph(x, mask, op, border)
+    got = ndl_amd.morph(x, mask, op, border)
+    assert same_bits(got, want)
+    monkeypatch.setenv("NDL_LINE_MARCH", "0")
+    monkeypatch.setenv("NDL_ROW_PASS", "0")
+    assert same_bits(ndl_amd.morph(x, mask, op, border), want)
