@@ -209,8 +209,15 @@ int run_axis_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const n
     return NDL_OK;
 }
 
+// the corner rule of the last erode / dilate pass of a float image: the
+// original source and, per tensor dimension, first-tap offset and border
+struct MorphCorner {
+    const ndl_tensor* src;
+    const int32_t* coff;
+    const int32_t* cbord;
+};
 int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
-                 hipStream_t s, int op = 0);
+                 hipStream_t s, int op = 0, const MorphCorner* mc = nullptr);
 
 // per-axis chain through dense accumulator-typed intermediates
 int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
@@ -580,7 +587,7 @@ bool vec_fits(const ndl_tensor* in, const ndl_tensor* out, int xd, int v)
 
 // An axis other than the unit-stride one: the line march.
 bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_axis_kernel& ak, int accum,
-               LineArgs& la, LineKey& key, int op)
+               LineArgs& la, LineKey& key, int op, const MorphCorner* mc)
 {
     if (!env_int("NDL_LINE_MARCH", 1) || !one_axis_types(in, out, ak, accum, key, op)) return false;
     const int nd = in->ndim, xd = unit_dim(in, out);
@@ -614,11 +621,27 @@ bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_ax
     const int64_t xv = in->extent[xd] / v;
     int64_t lanes = xv;
     la.xv.init((uint32_t)xv);
+    la.X = (int32_t)in->extent[xd];
+    if (mc) {
+        la.corner = mc->src->data;
+        la.cxs = mc->src->stride[xd];
+        la.cox = mc->coff[xd];
+        la.cbx = mc->cbord[xd];
+        la.czs = mc->src->stride[zd];
+        la.coz = mc->coff[zd];
+        la.cbz = mc->cbord[zd];
+    }
     for (int e = 0; e < nd; e++) {
         if (e == xd || e == zd || in->extent[e] == 1) continue;
+        if (mc && in->extent[e] > 0x7fffffffLL) return false;  // folded as 32-bit
         la.bext[la.nb].init((uint32_t)std::min<int64_t>(in->extent[e], 0xffffffffLL));
         la.sb[la.nb] = in->stride[e];
         la.db[la.nb] = out->stride[e];
+        if (mc) {
+            la.cbs[la.nb] = mc->src->stride[e];
+            la.cob[la.nb] = mc->coff[e];
+            la.cbb[la.nb] = mc->cbord[e];
+        }
         lanes *= in->extent[e];
         la.nb++;
         if (lanes > 0xfffffe00LL) return false;
@@ -687,17 +710,17 @@ bool plan_row(const ndl_tensor* in, const ndl_tensor* out, int xd, const ndl_axi
 // -1 when neither applies, an error code otherwise.  op: 0 convolution,
 // 1 erode, 2 dilate.
 int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
-                 hipStream_t s, int op)
+                 hipStream_t s, int op, const MorphCorner* mc)
 {
     LineKey key;
     LineArgs la;
-    if (plan_line(in, out, axis, ak, accum, la, key, op)) {
+    if (plan_line(in, out, axis, ak, accum, la, key, op, mc)) {
         hipError_t e = launch_line_march(la, key, s);
         return e == hipSuccess ? NDL_OK : hip_fail(e, "line march launch");
     }
     RowArgs ra;
     uint32_t blocks = 0;
-    if (plan_row(in, out, axis, ak, accum, ra, key, blocks, op)) {
+    if (!mc && plan_row(in, out, axis, ak, accum, ra, key, blocks, op)) {
         hipError_t e = launch_row_pass(ra, key, blocks, s);
         return e == hipSuccess ? NDL_OK : hip_fail(e, "row pass launch");
     }
@@ -1158,9 +1181,10 @@ int run_morph_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const 
 {
     int64_t n = numel(in);
     if (n == 0) return NDL_OK;
-    // a box pass without the corner rule: the one-axis kernels
-    if (!corner) {
-        const int rc = try_one_axis(in, out, axis, k, NDL_ACCUM_F32, s, dilate ? 2 : 1);
+    // a box pass: the one-axis kernels (the corner rule on the line march only)
+    {
+        const MorphCorner mc{corner, coff_dim, cbord_dim};
+        const int rc = try_one_axis(in, out, axis, k, NDL_ACCUM_F32, s, dilate ? 2 : 1, corner ? &mc : nullptr);
         if (rc != -1) return rc;
     }
     MorphPassArgs a;

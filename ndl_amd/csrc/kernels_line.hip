@@ -96,13 +96,21 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
     const uint32_t l = live ? lane : a.lanes - 1;
     uint32_t b = a.xv.div(l);
     const uint32_t xv = l - b * a.xv.d;
-    int64_t so = (int64_t)xv * V, dof = (int64_t)xv * V;
+    int64_t so = (int64_t)xv * V, dof = (int64_t)xv * V, co = 0;
     for (int i = 0; i < a.nb; i++) {
         const uint32_t q = a.bext[i].div(b);
         const uint32_t r = b - q * a.bext[i].d;
         so += (int64_t)r * a.sb[i];
         dof += (int64_t)r * a.db[i];
+        if constexpr (OP != 0) co += (int64_t)safe_map32(a.cbb[i], (int)a.bext[i].d, (int)r + a.cob[i]) * a.cbs[i];
         b = q;
+    }
+    // corner offsets of this lane's x elements (erode / dilate, last float pass)
+    int64_t cxo[OP != 0 ? V : 1];
+    if constexpr (OP != 0) {
+#pragma unroll
+        for (int e = 0; e < V; e++)
+            cxo[e] = co + (int64_t)safe_map32(a.cbx, a.X, (int)xv * V + e + a.cox) * a.cxs;
     }
     const TIn* scol = (const TIn*)a.src + so;
     TOut* dcol = (TOut*)a.dst + dof;
@@ -148,6 +156,16 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
                             if (morph_take<OP>(best, v)) best = v;
                         }
                         o[e] = best;
+                    }
+                    if constexpr (std::is_floating_point<TIn>::value) {
+                        if (a.corner) {
+                            const int zc = safe_map32(a.cbz, a.Z, z0 + i - (K - 1) + a.coz);
+#pragma unroll
+                            for (int e = 0; e < V; e++) {
+                                const TIn cv = ((const TIn*)a.corner)[cxo[e] + (int64_t)zc * a.czs];
+                                if (cv != cv) o[e] = cv;
+                            }
+                        }
                     }
                 }
                 if (live) store_v<TOut, V>(dcol + (int64_t)(z0 + i - (K - 1)) * a.dz, o);

@@ -247,6 +247,15 @@ struct LineArgs {
     int64_t db[NDL_MAX_DIM];
     float w[NDL_MAX_TAPS];   // NDL_ACCUM_F32
     double wd[NDL_MAX_TAPS]; // NDL_ACCUM_REF
+    // last erode / dilate pass of a float image: the original source, whose
+    // value at the structuring element's corner (per dimension: first-tap
+    // offset, border) replaces the result when it is NaN; null otherwise
+    const void* corner;
+    int32_t X;                              // x extent
+    int64_t cxs, czs;                       // corner strides of x and z
+    int32_t cox, cbx, coz, cbz;             // corner offsets / borders of x and z
+    int64_t cbs[NDL_MAX_DIM];               // ... of the batch dimensions
+    int32_t cob[NDL_MAX_DIM], cbb[NDL_MAX_DIM];
 };
 // One-axis kernel selection (erode / dilate: float32 or uint8, the same type
 // in and out, every tap included): storage types (NDL_F32 / NDL_F64 in and out),
