@@ -2,7 +2,9 @@
 // bf16 / f16 storage with FP32 accumulation): a 1-D
 // kernel image {k, 1, 1}, {1, k, 1}, ... given to convolve()
 // (processing/convolution.h:52-71), or one per-axis pass of gaussian_blur()
-// (convolution.h:107-186, through FP64 intermediates).
+// (convolution.h:107-186, through FP64 intermediates).  With OP = 1 / 2 the
+// same kernels run the per-axis min / max passes of erode / dilate over a
+// box (processing/morphology.h:57-101) on float32 and uint8 images.
 //
 // Arithmetic (this file is built with -ffp-contract=off):
 //  * Acc = double (NDL_ACCUM_REF): total = 0, then total = total +
