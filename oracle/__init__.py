@@ -52,6 +52,8 @@ def _load():
     abi.bind(lib, "oracle_bf16_to_double", C.c_double, C.c_uint16)
     abi.bind(lib, "oracle_double_to_bf16", C.c_uint16, C.c_double)
     abi.bind(lib, "oracle_cvtt_i32", C.c_int32, C.c_double)
+    abi.bind(lib, "oracle_set_threads", None, C.c_int)
+    abi.bind(lib, "oracle_get_threads", C.c_int)
     return lib
 
 
@@ -72,6 +74,24 @@ def _load_ref():
 
 lib = _load()
 ref_lib = _load_ref()
+
+
+class threads:
+    """with oracle.threads(n): run the oracle's passes on n host threads.  The
+    passes split independent outputs (and transposed scatters line by line),
+    so results are bit-identical to one thread."""
+
+    def __init__(self, n=None):
+        self.n = int(n or os.cpu_count() or 1)
+
+    def __enter__(self):
+        self.saved = lib.oracle_get_threads()
+        lib.oracle_set_threads(self.n)
+        return self
+
+    def __exit__(self, *exc):
+        lib.oracle_set_threads(self.saved)
+        return False
 
 
 class OracleError(RuntimeError):

@@ -46,6 +46,7 @@
  * compiled reference itself (oracle/_ref).
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -53,6 +54,60 @@
 #include "../include/ndl_gpu.h"
 
 #define EXPORT __attribute__((visibility("default")))
+
+/* ---------------------------------------------------------------- threads
+ * The passes below compute every output element independently (gathers) or
+ * one line at a time in the serial order (the transposed scatter), so
+ * splitting their index range over threads changes no result bit.  Default
+ * one thread: the timed CPU baseline is the reference's serial loop. */
+static int g_threads = 1;
+
+EXPORT void oracle_set_threads(int n) { g_threads = n < 1 ? 1 : (n > 256 ? 256 : n); }
+EXPORT int oracle_get_threads(void) { return g_threads; }
+
+typedef void (*range_fn)(void* ctx, int64_t lo, int64_t hi);
+struct range_job { range_fn fn; void* ctx; int64_t lo, hi; };
+
+static void* range_thread(void* p)
+{
+    struct range_job* j = (struct range_job*)p;
+    j->fn(j->ctx, j->lo, j->hi);
+    return NULL;
+}
+
+/* fn(ctx, lo, hi) over [0, n) in g_threads contiguous ranges */
+static void run_ranges(int64_t n, range_fn fn, void* ctx)
+{
+    int nt = g_threads;
+    if (nt > 1 && n < 4096) nt = 1;
+    if (nt <= 1) {
+        fn(ctx, 0, n);
+        return;
+    }
+    struct range_job jobs[256];
+    pthread_t tid[256];
+    int started[256];
+    for (int i = 0; i < nt; i++) {
+        jobs[i].fn = fn;
+        jobs[i].ctx = ctx;
+        jobs[i].lo = n * i / nt;
+        jobs[i].hi = n * (i + 1) / nt;
+        started[i] = i > 0 && pthread_create(&tid[i], NULL, range_thread, &jobs[i]) == 0;
+    }
+    for (int i = 0; i < nt; i++)
+        if (!started[i]) fn(ctx, jobs[i].lo, jobs[i].hi);
+    for (int i = 1; i < nt; i++)
+        if (started[i]) pthread_join(tid[i], NULL);
+}
+
+/* coordinates (dimension 0 fastest) of linear index lin */
+static void unravel(int64_t lin, int nd, const int64_t* extent, int64_t* coord)
+{
+    for (int d = 0; d < nd; d++) {
+        coord[d] = extent[d] ? lin % extent[d] : 0;
+        lin = extent[d] ? lin / extent[d] : 0;
+    }
+}
 
 /* ---------------------------------------------------------------- scalars */
 
@@ -257,32 +312,48 @@ EXPORT int oracle_convolve(const ndl_tensor* src, const ndl_tensor* dst, const d
 }
 
 /* one axis pass: out = sum_t k[t] * in[B(i + t - c)], FP64, ascending nonzero taps */
-static void axis_pass(const void* in, int in_dt, const int64_t* in_stride, void* out, int out_dt,
-                      const int64_t* out_stride, int nd, const int64_t* extent, int axis,
-                      const ndl_axis_kernel* k, int rounding)
+struct pass_ctx {
+    const void* in; int in_dt; const int64_t* in_stride;
+    void* out; int out_dt; const int64_t* out_stride;
+    int nd; const int64_t* extent; int axis; const ndl_axis_kernel* k; int rounding;
+};
+
+static void axis_pass_range(void* p, int64_t lo, int64_t hi)
 {
-    int64_t m = extent[axis];
-    int64_t n = 1;
-    for (int d = 0; d < nd; d++) n *= extent[d];
+    const struct pass_ctx* c = (const struct pass_ctx*)p;
+    const int nd = c->nd, axis = c->axis;
+    const ndl_axis_kernel* k = c->k;
+    int64_t m = c->extent[axis];
     int64_t coord[NDL_MAX_DIM] = {0};
-    for (int64_t lin = 0; lin < n; lin++) {
+    unravel(lo, nd, c->extent, coord);
+    for (int64_t lin = lo; lin < hi; lin++) {
         int64_t base = 0, doff = 0;
         for (int d = 0; d < nd; d++) {
-            if (d != axis) base += coord[d] * in_stride[d];
-            doff += coord[d] * out_stride[d];
+            if (d != axis) base += coord[d] * c->in_stride[d];
+            doff += coord[d] * c->out_stride[d];
         }
         double total = 0;
         for (int t = 0; t < k->count; t++) {
             if (k->taps[t] == 0.0) continue;
             int64_t x = border_map(k->border, m, coord[axis] + t - k->center);
-            total += load_elem(in, in_dt, base + x * in_stride[axis]) * k->taps[t];
+            total += load_elem(c->in, c->in_dt, base + x * c->in_stride[axis]) * k->taps[t];
         }
-        store_elem(out, out_dt, doff, total, rounding);
+        store_elem(c->out, c->out_dt, doff, total, c->rounding);
         for (int d = 0; d < nd; d++) {
-            if (++coord[d] < extent[d]) break;
+            if (++coord[d] < c->extent[d]) break;
             coord[d] = 0;
         }
     }
+}
+
+static void axis_pass(const void* in, int in_dt, const int64_t* in_stride, void* out, int out_dt,
+                      const int64_t* out_stride, int nd, const int64_t* extent, int axis,
+                      const ndl_axis_kernel* k, int rounding)
+{
+    struct pass_ctx c = {in, in_dt, in_stride, out, out_dt, out_stride, nd, extent, axis, k, rounding};
+    int64_t n = 1;
+    for (int d = 0; d < nd; d++) n *= extent[d];
+    run_ranges(n, axis_pass_range, &c);
 }
 
 static int check_axes(const ndl_tensor* t, int naxes, const int32_t* axes, const ndl_axis_kernel* k)
@@ -344,27 +415,51 @@ EXPORT int oracle_sepconv(const ndl_tensor* src, const ndl_tensor* dst, int naxe
     return NDL_OK;
 }
 
-/* transpose of one FP64 axis pass: out[B(i+t-c)] += k[t] * in[i] (dense buffers) */
-static void axis_pass_transpose(const double* in, double* out, int nd, const int64_t* extent,
-                                const int64_t* stride, int axis, const ndl_axis_kernel* k)
+/* transpose of one FP64 axis pass: out[B(i+t-c)] += k[t] * in[i] (dense
+ * buffers).  Every scatter stays inside its line along the axis, and a line
+ * is processed in ascending position order, so threads split the lines. */
+struct tpass_ctx {
+    const double* in; double* out; int nd; const int64_t* extent; const int64_t* stride;
+    int axis; const ndl_axis_kernel* k;
+};
+
+static void axis_pass_transpose_lines(void* p, int64_t lo, int64_t hi)
 {
-    int64_t m = extent[axis], n = 1;
-    for (int d = 0; d < nd; d++) n *= extent[d];
-    memset(out, 0, sizeof(double) * (size_t)n);
-    int64_t coord[NDL_MAX_DIM] = {0};
-    for (int64_t lin = 0; lin < n; lin++) {
-        int64_t base = lin - coord[axis] * stride[axis];
-        double g = in[lin];
-        for (int t = 0; t < k->count; t++) {
-            if (k->taps[t] == 0.0) continue;
-            int64_t x = border_map(k->border, m, coord[axis] + t - k->center);
-            out[base + x * stride[axis]] += k->taps[t] * g;
+    const struct tpass_ctx* c = (const struct tpass_ctx*)p;
+    const int nd = c->nd, axis = c->axis;
+    const ndl_axis_kernel* k = c->k;
+    int64_t m = c->extent[axis], ext[NDL_MAX_DIM], coord[NDL_MAX_DIM] = {0};
+    int od = 0;
+    for (int d = 0; d < nd; d++)
+        if (d != axis) ext[od++] = c->extent[d];
+    unravel(lo, od, ext, coord);
+    for (int64_t line = lo; line < hi; line++) {
+        int64_t base = 0;
+        for (int d = 0, j = 0; d < nd; d++)
+            if (d != axis) base += coord[j++] * c->stride[d];
+        for (int64_t i = 0; i < m; i++) {
+            double g = c->in[base + i * c->stride[axis]];
+            for (int t = 0; t < k->count; t++) {
+                if (k->taps[t] == 0.0) continue;
+                int64_t x = border_map(k->border, m, i + t - k->center);
+                c->out[base + x * c->stride[axis]] += k->taps[t] * g;
+            }
         }
-        for (int d = 0; d < nd; d++) {
-            if (++coord[d] < extent[d]) break;
+        for (int d = 0; d < od; d++) {
+            if (++coord[d] < ext[d]) break;
             coord[d] = 0;
         }
     }
+}
+
+static void axis_pass_transpose(const double* in, double* out, int nd, const int64_t* extent,
+                                const int64_t* stride, int axis, const ndl_axis_kernel* k)
+{
+    int64_t n = 1;
+    for (int d = 0; d < nd; d++) n *= extent[d];
+    memset(out, 0, sizeof(double) * (size_t)n);
+    struct tpass_ctx c = {in, out, nd, extent, stride, axis, k};
+    run_ranges(extent[axis] ? n / extent[axis] : 0, axis_pass_transpose_lines, &c);
 }
 
 static int gather_dense(const ndl_tensor* t, double* out)
@@ -422,6 +517,60 @@ EXPORT int oracle_sepconv_adjoint_input(const ndl_tensor* g, const ndl_tensor* g
     return NDL_OK;
 }
 
+/* one tap's sum, in the serial position order (dimension 0 fastest) */
+struct tap_ctx {
+    const double* gd; const double* u; int nd; const int64_t* extent; const int64_t* ds; int ax;
+    const ndl_axis_kernel* k; double* out;
+};
+
+static void tap_sum_one(void* p, int64_t t0, int64_t t1)
+{
+    const struct tap_ctx* c = (const struct tap_ctx*)p;
+    int64_t n = 1;
+    for (int d = 0; d < c->nd; d++) n *= c->extent[d];
+    const int ax = c->ax;
+    const int64_t m = c->extent[ax];
+    for (int64_t t = t0; t < t1; t++) {
+        double acc = 0;
+        int64_t coord[NDL_MAX_DIM] = {0};
+        for (int64_t lin = 0; lin < n; lin++) {
+            int64_t base = lin - coord[ax] * c->ds[ax];
+            int64_t xi = border_map(c->k->border, m, coord[ax] + t - c->k->center);
+            acc += c->gd[lin] * c->u[base + xi * c->ds[ax]];
+            for (int d = 0; d < c->nd; d++) {
+                if (++coord[d] < c->extent[d]) break;
+                coord[d] = 0;
+            }
+        }
+        c->out[t] = acc;
+    }
+}
+
+/* fn(ctx, i, i+1) for every i < n, one item per thread (up to g_threads) */
+static void run_ranges_each(int64_t n, range_fn fn, void* ctx)
+{
+    int saved = g_threads;
+    int64_t done = 0;
+    while (done < n) {
+        int64_t chunk = n - done < saved ? n - done : saved;
+        struct range_job jobs[256];
+        pthread_t tid[256];
+        int started[256];
+        for (int64_t i = 0; i < chunk; i++) {
+            jobs[i].fn = fn;
+            jobs[i].ctx = ctx;
+            jobs[i].lo = done + i;
+            jobs[i].hi = done + i + 1;
+            started[i] = i > 0 && pthread_create(&tid[i], NULL, range_thread, &jobs[i]) == 0;
+        }
+        for (int64_t i = 0; i < chunk; i++)
+            if (!started[i]) fn(ctx, jobs[i].lo, jobs[i].hi);
+        for (int64_t i = 1; i < chunk; i++)
+            if (started[i]) pthread_join(tid[i], NULL);
+        done += chunk;
+    }
+}
+
 /* dL/dk_a[t] = sum_p g[p] * U_a[B_a(p + t - c_a)], U_a = x through every other
  * axis (FP64, axis order as given), p in coordinate order. */
 EXPORT int oracle_sepconv_adjoint_taps(const ndl_tensor* x, const ndl_tensor* g, int naxes,
@@ -450,20 +599,8 @@ EXPORT int oracle_sepconv_adjoint_taps(const ndl_tensor* x, const ndl_tensor* g,
         }
         int ax = axes[a];
         int64_t m = x->extent[ax];
-        for (int t = 0; t < k[a].count; t++) {
-            double acc = 0;
-            int64_t coord[NDL_MAX_DIM] = {0};
-            for (int64_t lin = 0; lin < n; lin++) {
-                int64_t base = lin - coord[ax] * ds[ax];
-                int64_t xi = border_map(k[a].border, m, coord[ax] + t - k[a].center);
-                acc += gd[lin] * u[base + xi * ds[ax]];
-                for (int d = 0; d < nd; d++) {
-                    if (++coord[d] < x->extent[d]) break;
-                    coord[d] = 0;
-                }
-            }
-            grad[off + t] = acc;
-        }
+        struct tap_ctx c = {gd, u, nd, x->extent, ds, ax, &k[a], grad + off};
+        run_ranges_each(k[a].count, tap_sum_one, &c);
         off += k[a].count;
     }
     free(u);

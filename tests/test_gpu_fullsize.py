@@ -1,0 +1,287 @@
+"""GPU parity at the benchmark configurations' full sizes.
+
+The oracle runs its passes on every host core here (oracle.threads): each
+output is computed independently and transposed scatters stay in their line,
+so the results are bit-identical to the serial oracle.
+
+  * config 2a: 256^3 float32 Gaussian 11/11/19 reflect, reference arithmetic
+    bit-exact and FP32 within 1e-6 relative;
+  * config 3: 1920x1080x3x32 float32 video (x, y sigma 1 clamp, t binomial
+    wrap), FP32 within 1e-6 of sum|w||x| and reference arithmetic bit-exact;
+  * config 4: all 256 uneven instances in one plan, once with the z chunk
+    forced (work-queue variant, every group of the full plan runs), once
+    autotuned, each instance against the oracle, over three graph replays;
+  * config 5: tap gradients with bf16 / f16 storage of x at 384^3 against
+    the FP64 oracle and FP64 central differences (step 1e-3) of the
+    forward, and the input adjoint at 64 random voxels against central
+    differences.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import ndl_amd
+import oracle
+from ndl_amd import abi, workloads
+
+pytestmark = pytest.mark.gpu
+
+REF, F32 = abi.ACCUM_REF, abi.ACCUM_F32
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_sep(x, axes, taps, borders, accum, dst_dtype=None, bf16=False):
+    d = ndl_amd.DeviceArray.from_numpy(x, bf16=bf16)
+    out = ndl_amd.DeviceArray(x.shape, dst_dtype or x.dtype, bf16=bf16 and dst_dtype is None)
+    ndl_amd.sepconv_dev(d, out, axes, taps, borders, accum)
+    r = out.to_numpy()
+    d.free()
+    out.free()
+    return r
+
+
+def test_config2a_256_reference_bit_exact():
+    x, axes, taps, borders = workloads.config2a(n=256)
+    say("config2a 256^3: oracle")
+    with oracle.threads():
+        want = oracle.sepconv(x, axes, taps, borders)
+    say("config2a 256^3: REF on the GPU")
+    np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, REF).view(np.uint32), want.view(np.uint32))
+    got = gpu_sep(x, axes, taps, borders, F32)
+    err = float(np.max(np.abs(got.astype(np.float64) - want) / np.abs(want.astype(np.float64))))
+    say("  fp32 max rel err", err)
+    assert err <= 1e-6
+
+
+def test_config3_full_size():
+    say("config3 1920x1080x3x32: inputs")
+    x, axes, taps, borders = workloads.config3()
+    say("config3: oracle")
+    with oracle.threads():
+        want = oracle.sepconv(x, axes, taps, borders)
+    say("config3: FP32 on the GPU")
+    got = gpu_sep(x, axes, taps, borders, F32)
+    # signed data (U[0,1) + N(0, 0.05^2)): error relative to sum|w||x|
+    scale = float(np.max(np.abs(x))) * float(np.prod([np.sum(np.abs(t)) for t in taps]))
+    err = float(np.max(np.abs(got.astype(np.float64) - want.astype(np.float64)))) / scale
+    say("  fp32 max err / (max|x| prod sum|w|)", err)
+    assert err <= 1e-6
+    del got
+    say("config3: REF on the GPU")
+    got = gpu_sep(x, axes, taps, borders, REF)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def _config4_plan(items, values):
+    srcs, dsts, bitems, keep = [], [], [], []
+    for (shape, taps, border), x in zip(items, values):
+        nd = len(shape)
+        sd = ndl_amd.DeviceArray.from_numpy(x)
+        dd = ndl_amd.DeviceArray(shape, np.float32)
+        ks = abi.Kernels(taps, [border] * nd)
+        it = abi.BatchItem()
+        it.src, it.dst, it.naxes = sd.tensor(), dd.tensor(), nd
+        for a in range(nd):
+            it.axes[a] = a
+            it.k[a] = ks.arr[a]
+        srcs.append(sd)
+        dsts.append(dd)
+        bitems.append(it)
+        keep.append(ks)
+    arr = (abi.BatchItem * len(bitems))(*bitems)
+    plan = C.c_void_p()
+    ndl_amd.check(ndl_amd.lib().ndl_plan_create(arr, len(bitems), F32, C.byref(plan)))
+    return plan, srcs, dsts, (arr, keep)
+
+
+@pytest.mark.parametrize("variant", ["work_queue_tz32", "autotuned"])
+def test_config4_full_plan(variant, monkeypatch):
+    items = workloads.config4()
+    say("config4", variant, ": inputs and oracle for 256 instances")
+    values = [workloads.config4_values(5, i, shape) for i, (shape, _, _) in enumerate(items)]
+    wants = []
+    with oracle.threads():
+        for (shape, taps, border), x in zip(items, values):
+            nd = len(shape)
+            wants.append(oracle.sepconv(x, list(range(nd)), taps, [border] * nd))
+    if variant == "work_queue_tz32":
+        # one candidate, the persistent work queues: every (tile shape, ring
+        # period) group of the full plan runs, and nothing overwrites it
+        monkeypatch.setenv("NDL_PLAN_TZ", "32")
+    plan, srcs, dsts, keep = _config4_plan(items, values)
+    lib = ndl_amd.lib()
+    try:
+        for rep in range(3):  # graph replays, with host work and syncs between
+            for d in dsts:
+                ndl_amd.check(lib.ndl_memset(d.ptr, 0, d.nbytes, None))
+            say("  execute", rep)
+            ndl_amd.check(lib.ndl_plan_execute(plan, None))
+            ndl_amd.synchronize()
+            worst = 0.0
+            for i, (d, want) in enumerate(zip(dsts, wants)):
+                got = d.to_numpy().astype(np.float64)
+                w64 = want.astype(np.float64)
+                err = float(np.max(np.abs(got - w64) / np.abs(w64)))
+                worst = max(worst, err)
+                assert err <= 1e-6, (rep, i, items[i][0], items[i][2], [len(t) for t in items[i][1]], err)
+            say("  max rel err over 256 instances", worst)
+    finally:
+        ndl_amd.check(lib.ndl_plan_destroy(plan))
+
+
+def _abs_bound(xd, w, axes, taps, borders):
+    """sum_p |g_p| Ubar_p per tap (Ubar = |x| through |taps|): the worst-case
+    scale of an error of the U values relative to Ubar"""
+    return oracle.adjoint_taps(np.abs(xd), np.abs(w.astype(np.float64)), axes, [np.abs(t) for t in taps], borders)
+
+
+def _rms_bound(xd, w, axes, taps, borders):
+    """an upper bound of sqrt(sum_p g_p^2 Ubar_p^2) per tap: by Cauchy-Schwarz
+    Ubar^2 <= (prod over the other axes of sum|k|) * (x^2 through |taps|).
+    Rounding errors of U with random signs add up to about this times eps."""
+    s = oracle.adjoint_taps(xd ** 2, w.astype(np.float64) ** 2, axes, [np.abs(t) for t in taps], borders)
+    sums = [float(np.abs(t).sum()) for t in taps]
+    fac = np.concatenate([np.full(len(t), np.prod([sums[b] for b in range(len(taps)) if b != a]))
+                          for a, t in enumerate(taps)])
+    return np.sqrt(s * fac)
+
+
+@pytest.mark.parametrize("gdt", ["same", "f32"])
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("border", ["clamp", "reflect"])
+def test_config5_half_storage_tap_gradients_small(border, storage, gdt):
+    # config-5 shape at 40^3: x in bf16 / f16 storage, grad_out either in
+    # x's storage type (the fused tap-gradient route) or float32 (the chain
+    # route); FP64 oracle on the storage-rounded values; bound: 4e-6 of the
+    # abs-weighted sum (FP32 U values, FP64 accumulation)
+    xs, xd, w, taps = workloads.config5(n=40, storage=storage)
+    bf = storage == "bf16"
+    axes = [0, 1, 2]
+    borders = [border] * 3
+    if gdt == "same":
+        ws, wd = (workloads.config5(n=40, seed=16, storage=storage)[:2])
+        g_dev = ndl_amd.DeviceArray.from_numpy(ws, bf16=bf)
+    else:
+        wd = w.astype(np.float64)
+        g_dev = ndl_amd.DeviceArray.from_numpy(w)
+    say("config5 small tap gradients", border, storage, gdt)
+    want = oracle.adjoint_taps(xd, wd, axes, taps, borders)
+    bound = _abs_bound(xd, wd, axes, taps, borders)
+    rms = _rms_bound(xd, wd, axes, taps, borders)
+    dx = ndl_amd.DeviceArray.from_numpy(xs, bf16=bf)
+    out = ndl_amd.DeviceArray((27,), np.float64)
+    ndl_amd.adjoint_taps_dev(dx, g_dev, axes, taps, borders, out, F32)
+    got = out.to_numpy()
+    rel = np.abs(got - want) / bound
+    say("  max err / abs-weighted sum", float(rel.max()), "max err / rms bound", float(np.max(np.abs(got - want) / rms)))
+    assert np.all(rel <= 4e-6)
+    assert np.all(np.abs(got - want) <= 100 * 2.0 ** -24 * rms)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_config5_full_size_gradients_vs_central_differences(storage):
+    # 384^3, reflect (symmetric-index fold) on every axis, 3 x 9 random taps:
+    # tap gradients of L = sum(w * conv(x)) with x in half storage, against
+    # (a) the FP64 oracle gradient and (b) FP64 central differences with
+    # step 1e-3 of the forward (reference arithmetic, FP64 output, on the
+    # storage-rounded x); input adjoint at 64 random voxels against central
+    # differences of the same forward.  L is linear in x and in each tap,
+    # so central differences are exact up to FP64 rounding.
+    n = 384
+    xs, xd, w, taps = workloads.config5(n=n, storage=storage)
+    bf = storage == "bf16"
+    axes = [0, 1, 2]
+    borders = ["reflect"] * 3
+    w64 = w.astype(np.float64)
+    say("config5 384^3", storage, ": FP64 oracle tap gradient")
+    with oracle.threads():
+        want = oracle.adjoint_taps(xd, w64, axes, taps, borders)
+        bound = _abs_bound(xd, w, axes, taps, borders)
+        rms = _rms_bound(xd, w, axes, taps, borders)
+    say("config5: GPU tap gradient (FP32 accumulation)")
+    dx = ndl_amd.DeviceArray.from_numpy(xs, bf16=bf)
+    dw = ndl_amd.DeviceArray.from_numpy(w)
+    out = ndl_amd.DeviceArray((27,), np.float64)
+    ndl_amd.adjoint_taps_dev(dx, dw, axes, taps, borders, out, F32)
+    got = out.to_numpy()
+    err = got - want
+    say("  vs oracle: max |err| / abs-weighted sum", float(np.max(np.abs(err) / bound)),
+        "max |err| / rms bound", float(np.max(np.abs(err) / rms)),
+        "max abs err", float(np.max(np.abs(err))), "rms err", float(np.sqrt(np.mean(err ** 2))),
+        "rms gradient", float(np.sqrt(np.mean(want ** 2))))
+    # worst case (cannot fail from rounding alone) and the statistical scale
+    # with a 100x margin over one FP32 rounding per value
+    assert np.all(np.abs(err) <= 4e-6 * bound)
+    assert np.all(np.abs(err) <= 100 * 2.0 ** -24 * rms)
+    dx.free()
+
+    # (b) central differences of L over every tap: FP64 forward on the GPU in
+    # the reference arithmetic, FP64 output, dot with w on the host
+    h = 1e-3
+    d64 = ndl_amd.DeviceArray.from_numpy(xd)
+    y64 = ndl_amd.DeviceArray(xd.shape, np.float64)
+
+    def loss(tp):
+        ndl_amd.sepconv_dev(d64, y64, axes, tp, borders, REF)
+        return float(np.dot(y64.to_numpy().ravel(), w64.ravel()))
+
+    fd = np.zeros(27)
+    for a in range(3):
+        for t in range(9):
+            tp = [np.array(v, dtype=np.float64) for v in taps]
+            tp[a][t] = taps[a][t] + h
+            lp = loss(tp)
+            tp[a][t] = taps[a][t] - h
+            lm = loss(tp)
+            fd[a * 9 + t] = (lp - lm) / (2 * h)
+        say("  central differences, axis", a, "done")
+    efd = got - fd
+    say("  vs central differences: max |err| / abs-weighted sum", float(np.max(np.abs(efd) / bound)))
+    # FD itself: FP64 rounding of L (|L| <= sum|w||y|) over 2h
+    # (about 27 roundings per output and the dot product's ~26 levels, each
+    # eps = 1.1e-16, relative to sum|w||y| <= lscale; doubled over 2h)
+    lscale = float(np.sum(np.abs(w64) * np.abs(xd))) * float(np.prod([np.abs(t).sum() for t in taps]))
+    fd_tol = 1e-14 * lscale / h
+    say("  oracle vs central differences: max |diff|", float(np.max(np.abs(want - fd))), "tolerance", fd_tol)
+    assert np.all(np.abs(want - fd) <= fd_tol)
+    assert np.all(np.abs(efd) <= 100 * 2.0 ** -24 * rms + fd_tol)
+
+    # (c) input adjoint at 64 random voxels: perturb all of them at once (+h,
+    # then -h); their output boxes (radius 2r = 8 per axis) are disjoint, so
+    # the change of L summed over voxel i's box is voxel i's central
+    # difference
+    say("config5: input adjoint vs central differences at 64 voxels")
+    gx = ndl_amd.DeviceArray(w.shape, np.float32)
+    ndl_amd.adjoint_input_dev(dw, gx, axes, taps, borders, F32)
+    gxv = gx.to_numpy().astype(np.float64)
+    rng = np.random.default_rng(64)
+    cells = rng.permutation(20 ** 3)[:64]
+    pts = []
+    for c in cells:
+        iz, iy, ix = c // 400, (c // 20) % 20, c % 20
+        base = np.array([iz, iy, ix]) * 19 + 1
+        pts.append(tuple(int(v) for v in base + rng.integers(0, 2, 3)))
+    ys = []
+    for sgn in (1.0, -1.0):
+        xp = xd.copy()
+        for p in pts:
+            xp[p] += sgn * h
+        d64 = ndl_amd.DeviceArray.from_numpy(xp)
+        ndl_amd.sepconv_dev(d64, y64, axes, taps, borders, REF)
+        ys.append(y64.to_numpy())
+    dy = ys[0] - ys[1]
+    worst = 0.0
+    for p in pts:
+        box = tuple(slice(max(0, q - 8), min(n, q + 9)) for q in p)
+        fdv = float(np.sum(w64[box] * dy[box])) / (2 * h)
+        scale = float(np.sum(np.abs(w64[box]))) * float(np.prod([np.abs(t).sum() for t in taps]))
+        e = abs(gxv[p] - fdv) / scale
+        worst = max(worst, e)
+        assert e <= 2e-6, (p, gxv[p], fdv)
+    say("  max |gx - fd| / (sum|w| prod sum|k|) over 64 voxels", worst)
