@@ -195,6 +195,24 @@ int ndl_sepconv_adjoint_taps(const ndl_tensor* x, const ndl_tensor* grad_out, in
                              const int32_t* axes, const ndl_axis_kernel* k, double* grad_taps,
                              int32_t accum, void* stream);
 
+/* ---- allocation-free input adjoint (no counterpart in the reference) -----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The result
+ * of ndl_sepconv_adjoint_input(..., NDL_ACCUM_F32, ...) when that adjoint
+ * runs as one fused pass: grad_in of grad_out's type (f32/f16/bf16), at most
+ * three convolved axes, no zero tap, a unit-stride dimension in both.  The
+ * edge weight rows of clamp / asymmetric-reflect axes go to a library-owned
+ * device array of NDL_NOALLOC_SCRATCH_BYTES on each device (held until the
+ * process ends); nothing is allocated or freed.  Returns NDL_ERR_ARG when the
+ * call would need the per-axis chain (use ndl_sepconv_adjoint_input) and
+ * NDL_ERR_NOMEM when the rows exceed the array.  Every call on one device
+ * shares that array: calls that overlap in time on one device (other
+ * streams, other host threads) race. */
+#define NDL_NOALLOC_SCRATCH_BYTES (256 * 1024)
+int ndl_sepconv_adjoint_input_noalloc(const ndl_tensor* grad_out, const ndl_tensor* grad_in,
+                                      int32_t naxes, const int32_t* axes, const ndl_axis_kernel* k,
+                                      void* stream);
+
 /* ---- Gaussian taps --------------------------------------------------------
  * The taps gaussian_blur() builds (convolution.h:116-126): radius
  * _kernelSize(sigma) (mathHelpers.h:17-18), w_i = exp(-d^2/(2 sigma^2)),

@@ -44,6 +44,7 @@ SIGNATURES = {
     "ndl_gaussian_scales": (C.c_int, [_T, _T, C.c_int32, _DP, C.c_int32, C.c_int32, C.c_int32, _VP]),
     "ndl_dog": (C.c_int, [_T, _T, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_input": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
+    "ndl_sepconv_adjoint_input_noalloc": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, _VP]),
     "ndl_sepconv_adjoint_taps": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, _VP, C.c_int32, _VP]),
     "ndl_gaussian_taps": (C.c_int, [C.c_double, _DP, C.c_int32]),
     "ndl_plan_create": (C.c_int, [_P(abi.BatchItem), C.c_int32, C.c_int32, _P(_VP)]),

@@ -801,83 +801,121 @@ void build_adjoint_axis(const ndl_axis_kernel& k, int64_t m, AdjAxis& out)
     out.nw = (size_t)(out.EL + out.ER) * K;
 }
 
+// edge weight rows of the table axes `which` into dst[q] (float or double
+// rows), built on the device (no host buffer has to outlive the call)
+int fill_edge_tables(const ndl_tensor* g, const int32_t* axes, const ndl_axis_kernel* k,
+                     const std::vector<AdjAxis>& adj, const std::vector<int>& which, const std::vector<void*>& dst,
+                     int dt, hipStream_t s)
+{
+    for (size_t g0 = 0; g0 < which.size(); g0 += 3) {
+        EdgeTables et;
+        memset(&et, 0, sizeof et);
+        for (size_t q = g0; q < which.size() && q < g0 + 3; q++) {
+            const int i = which[q], e = et.naxes++;
+            et.K[e] = k[i].count;
+            et.c[e] = k[i].center;
+            et.border[e] = k[i].border;
+            et.EL[e] = adj[i].EL;
+            et.ER[e] = adj[i].ER;
+            et.M[e] = g->extent[axes[i]];
+            for (int t = 0; t < k[i].count; t++) et.taps[e][t] = k[i].taps[t];
+            if (dt == NDL_F64) et.outd[e] = (double*)dst[q];
+            else et.outf[e] = (float*)dst[q];
+        }
+        hipError_t e = launch_edge_tables(et, s);
+        if (e != hipSuccess) return hip_fail(e, "edge-table build");
+    }
+    return NDL_OK;
+}
+
+// The whole input adjoint as one fused pass (FP32 accumulation, grad_in of
+// grad_out's type): the band in the tile's taps, weight rows at the ends.
+// Table memory: `tabmem` (tabcap floats) when given, else a stream-ordered
+// allocation.  Returns 1 when launched, 0 when the fused kernel does not
+// apply, or a negative status.
+struct AdjointPlan {
+    std::vector<AdjAxis> adj;
+    std::vector<int32_t> rax;
+    std::vector<ndl_axis_kernel> rk;
+    bool any_table = false;
+};
+void plan_adjoint(const ndl_tensor* g, int32_t naxes, const int32_t* axes, const ndl_axis_kernel* k, AdjointPlan& ap)
+{
+    ap.adj.resize(naxes);
+    for (int i = 0; i < naxes; i++) {
+        build_adjoint_axis(k[i], g->extent[axes[i]], ap.adj[i]);
+        ap.any_table |= ap.adj[i].table;
+    }
+    // A = A_{n-1}..A_0, so A^T applies A_{n-1}^T first
+    ap.rax.resize(naxes);
+    ap.rk.resize(naxes);
+    for (int i = 0; i < naxes; i++) {
+        ap.rax[i] = axes[naxes - 1 - i];
+        ap.rk[i] = ap.adj[naxes - 1 - i].k;
+    }
+}
+
+int adjoint_fused_pass(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes, const int32_t* axes,
+                       const ndl_axis_kernel* k, const AdjointPlan& ap, int32_t accum, float* tabmem, size_t tabcap,
+                       hipStream_t s)
+{
+    if (g->dtype != gx->dtype) return 0;
+    FusedArgs fa;
+    FusedConfig cfg;
+    bool int_acc = false;
+    int rd[3];
+    if (!plan_fused(g, gx, naxes, ap.rax.data(), ap.rk.data(), accum, fa, cfg, int_acc, 0, ap.any_table, rd)) return 0;
+    std::vector<int> which, role;
+    size_t nfl = 0;
+    for (int r = 0; r < 3; r++)
+        for (int i = 0; rd[r] >= 0 && i < naxes; i++)
+            if (axes[i] == rd[r] && ap.adj[i].table) {
+                which.push_back(i);
+                role.push_back(r);
+                nfl += ap.adj[i].nw;
+            }
+    Scratch etab;
+    float* base = tabmem;
+    if (!which.empty()) {
+        if (tabmem) {
+            if (nfl > tabcap)
+                return fail(NDL_ERR_NOMEM, "edge weight rows need %zu floats; the allocation-free buffer holds %zu",
+                            nfl, tabcap);
+        } else {
+            int rc = alloc_scratch(etab, nfl * sizeof(float), s);
+            if (rc) return rc;
+            base = (float*)etab.p;
+        }
+    }
+    std::vector<void*> dst;
+    size_t off = 0;
+    for (size_t q = 0; q < which.size(); q++) {
+        float* p = base + off;
+        dst.push_back(p);
+        fa.etab[role[q]] = p;
+        fa.eL[role[q]] = ap.adj[which[q]].EL;
+        fa.eR[role[q]] = ap.adj[which[q]].ER;
+        off += ap.adj[which[q]].nw;
+    }
+    int rc = fill_edge_tables(g, axes, k, ap.adj, which, dst, NDL_F32, s);
+    if (rc) return rc;
+    hipError_t e = launch_fused(fa, cfg, g->dtype, int_acc, s);
+    if (e == hipSuccess) return 1;
+    if (e != hipErrorInvalidValue) return hip_fail(e, "fused adjoint launch");
+    return 0;
+}
+
 int adjoint_input_impl(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes, const int32_t* axes,
                        const ndl_axis_kernel* k, int32_t accum, hipStream_t s)
 {
-    std::vector<AdjAxis> adj(naxes);
-    bool any_table = false;
-    for (int i = 0; i < naxes; i++) {
-        build_adjoint_axis(k[i], g->extent[axes[i]], adj[i]);
-        any_table |= adj[i].table;
-    }
-    // A = A_{n-1}..A_0, so A^T applies A_{n-1}^T first
-    std::vector<int32_t> rax(naxes);
-    std::vector<ndl_axis_kernel> rk(naxes);
-    for (int i = 0; i < naxes; i++) {
-        rax[i] = axes[naxes - 1 - i];
-        rk[i] = adj[naxes - 1 - i].k;
-    }
-    if (!any_table) return forward_impl(g, gx, naxes, rax.data(), rk.data(), accum, NDL_ROUND_TRUNC, s);
+    AdjointPlan ap;
+    plan_adjoint(g, naxes, axes, k, ap);
+    const std::vector<AdjAxis>& adj = ap.adj;
+    if (!ap.any_table) return forward_impl(g, gx, naxes, ap.rax.data(), ap.rk.data(), accum, NDL_ROUND_TRUNC, s);
 
-    // edge weight rows of every table axis, built on the device (no host
-    // buffer has to outlive this call)
-    auto fill_tables = [&](const std::vector<int>& which, const std::vector<void*>& dst, int dt) -> int {
-        for (size_t g0 = 0; g0 < which.size(); g0 += 3) {
-            EdgeTables et;
-            memset(&et, 0, sizeof et);
-            for (size_t q = g0; q < which.size() && q < g0 + 3; q++) {
-                const int i = which[q], e = et.naxes++;
-                et.K[e] = k[i].count;
-                et.c[e] = k[i].center;
-                et.border[e] = k[i].border;
-                et.EL[e] = adj[i].EL;
-                et.ER[e] = adj[i].ER;
-                et.M[e] = g->extent[axes[i]];
-                for (int t = 0; t < k[i].count; t++) et.taps[e][t] = k[i].taps[t];
-                if (dt == NDL_F64) et.outd[e] = (double*)dst[q];
-                else et.outf[e] = (float*)dst[q];
-            }
-            hipError_t e = launch_edge_tables(et, s);
-            if (e != hipSuccess) return hip_fail(e, "edge-table build");
-        }
-        return NDL_OK;
-    };
-
-    // one fused pass: the band in the tile's taps, weight rows at the ends
-    if (g->dtype == gx->dtype) {
-        FusedArgs fa;
-        FusedConfig cfg;
-        bool int_acc = false;
-        int rd[3];
-        if (plan_fused(g, gx, naxes, rax.data(), rk.data(), accum, fa, cfg, int_acc, 0, true, rd)) {
-            std::vector<int> which, role;
-            size_t nfl = 0;
-            for (int r = 0; r < 3; r++)
-                for (int i = 0; rd[r] >= 0 && i < naxes; i++)
-                    if (axes[i] == rd[r] && adj[i].table) {
-                        which.push_back(i);
-                        role.push_back(r);
-                        nfl += adj[i].nw;
-                    }
-            Scratch etab;
-            int rc = alloc_scratch(etab, nfl * sizeof(float), s);
-            if (rc) return rc;
-            std::vector<void*> dst;
-            size_t off = 0;
-            for (size_t q = 0; q < which.size(); q++) {
-                float* p = (float*)etab.p + off;
-                dst.push_back(p);
-                fa.etab[role[q]] = p;
-                fa.eL[role[q]] = adj[which[q]].EL;
-                fa.eR[role[q]] = adj[which[q]].ER;
-                off += adj[which[q]].nw;
-            }
-            if ((rc = fill_tables(which, dst, NDL_F32))) return rc;
-            hipError_t e = launch_fused(fa, cfg, g->dtype, int_acc, s);
-            if (e == hipSuccess) return NDL_OK;
-            if (e != hipErrorInvalidValue) return hip_fail(e, "fused adjoint launch");
-        }
-    }
+    int fr = adjoint_fused_pass(g, gx, naxes, axes, k, ap, accum, nullptr, 0, s);
+    if (fr < 0) return fr;
+    if (fr == 1) return NDL_OK;
 
     const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
     const size_t acc_sz = dtype_size(acc_dt);
@@ -899,7 +937,7 @@ int adjoint_input_impl(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes,
         }
         off += adj[i].nw * acc_sz;
     }
-    if ((rc = fill_tables(which, dst, acc_dt))) return rc;
+    if ((rc = fill_edge_tables(g, axes, k, adj, which, dst, acc_dt, s))) return rc;
     if (naxes > 1) {
         size_t bytes = (size_t)numel(g) * acc_sz;
         for (int i = 0; i < 2 && i < naxes - 1; i++) {
@@ -1300,6 +1338,63 @@ NDL_API int ndl_sepconv_adjoint_input(const ndl_tensor* grad_out, const ndl_tens
     if ((rc = check_no_overlap(grad_out, grad_in))) return rc;
     if (numel(grad_out) == 0) return NDL_OK;
     return adjoint_input_impl(grad_out, grad_in, naxes, axes, k, accum, (hipStream_t)stream);
+}
+
+// ------------------------------------------- allocation-free input adjoint
+// The edge weight rows of ndl_sepconv_adjoint_input_noalloc: one array per
+// device, held until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is a stand-in, not a device address):
+// scratch_ptr() looks up each device's own address once, with that device
+// current, and every use goes through it.
+constexpr size_t kNoAllocFloats = NDL_NOALLOC_SCRATCH_BYTES / sizeof(float);
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_ptr(float** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(NDL_ERR_ARG, "device %d beyond the %d this library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch_dev[dev] = (float*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return NDL_OK;
+}
+} // namespace
+
+NDL_API int ndl_sepconv_adjoint_input_noalloc(const ndl_tensor* grad_out, const ndl_tensor* grad_in, int32_t naxes,
+                                              const int32_t* axes, const ndl_axis_kernel* k, void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(grad_out, "grad_out")) || (rc = check_tensor(grad_in, "grad_in"))) return rc;
+    if ((rc = check_same_shape(grad_out, grad_in, "grad_out", "grad_in"))) return rc;
+    if ((rc = check_axes(grad_out, naxes, axes, k))) return rc;
+    if (!(is_f32_family(grad_out->dtype) && is_f32_family(grad_in->dtype)))
+        return fail(NDL_ERR_DTYPE, "the allocation-free adjoint takes f32/f16/bf16 fields");
+    if ((rc = check_no_overlap(grad_out, grad_in))) return rc;
+    if (numel(grad_out) == 0) return NDL_OK;
+    float* tab = nullptr;
+    if ((rc = scratch_ptr(&tab))) return rc;
+    AdjointPlan ap;
+    plan_adjoint(grad_out, naxes, axes, k, ap);
+    int fr = adjoint_fused_pass(grad_out, grad_in, naxes, axes, k, ap, NDL_ACCUM_F32, tab, kNoAllocFloats,
+                                (hipStream_t)stream);
+    if (fr < 0) return fr;
+    if (fr == 0)
+        return fail(NDL_ERR_ARG, "this adjoint needs the per-axis chain, which allocates intermediates (no fused "
+                                 "kernel for these axes, taps and types: e.g. grad_in of another type, more than "
+                                 "three axes, a zero tap, weight rows on half storage); use ndl_sepconv_adjoint_input");
+    return NDL_OK;
 }
 
 NDL_API int ndl_sepconv_adjoint_taps(const ndl_tensor* x, const ndl_tensor* grad_out, int32_t naxes,
