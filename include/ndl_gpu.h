@@ -36,7 +36,7 @@ extern "C" {
 #endif
 
 #define NDL_MAX_DIM 8
-#define NDL_MAX_TAPS 64 /* per-axis tap limit of the separable entry points */
+#define NDL_MAX_TAPS 1024 /* per-axis tap limit of the separable entry points */
 
 /* element types (Image<T,DIM>::value_type of the reference) */
 enum ndl_dtype {

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 NDL_MAX_DIM = 8
-NDL_MAX_TAPS = 64
+NDL_MAX_TAPS = 1024
 
 U8, I8, U16, I16, I32, F32, F64, F16, BF16 = range(9)
 CLAMP, WRAP, REFLECT = 0, 1, 2

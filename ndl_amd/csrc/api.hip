@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -149,6 +150,64 @@ int alloc_scratch(Scratch& sc, size_t bytes, hipStream_t s)
     return NDL_OK;
 }
 
+// ------------------------------------------------ stream-ordered uploads
+// Small host arrays (tap lists) reach the device through pinned staging
+// slots owned by the library: the host data is copied into a slot, the slot
+// is copied to the device on the caller's stream, and an event recorded
+// after that copy marks when the slot may be reused.  No call waits for the
+// stream.  Under stream capture the slot is never reused (a graph replay
+// copies from it again), so captured calls keep their own slots.
+struct StagingSlot {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool recorded = false;
+};
+std::mutex g_stage_mu;
+std::vector<StagingSlot> g_stage;         // reusable slots
+std::vector<StagingSlot> g_stage_pinned;  // slots referenced by captured graphs
+
+int stage_upload(void* dev, const void* host, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return NDL_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(s, &cs);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamIsCapturing");
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    StagingSlot* slot = nullptr;
+    if (!capturing) {
+        for (StagingSlot& c : g_stage) {
+            if (c.cap < bytes) continue;
+            if (!c.recorded) { slot = &c; break; }
+            const hipError_t q = hipEventQuery(c.ev);
+            if (q == hipSuccess) { slot = &c; break; }
+            if (q != hipErrorNotReady) return hip_fail(q, "staging event");
+        }
+    }
+    if (!slot) {
+        StagingSlot c;
+        c.cap = std::max<size_t>(4096, (bytes + 4095) & ~(size_t)4095);
+        if ((e = hipHostMalloc(&c.p, c.cap, hipHostMallocDefault)) != hipSuccess)
+            return fail(NDL_ERR_NOMEM, "pinned staging of %zu bytes: %s", c.cap, hipGetErrorString(e));
+        if ((e = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming)) != hipSuccess) {
+            (void)hipHostFree(c.p);
+            return hip_fail(e, "staging event");
+        }
+        std::vector<StagingSlot>& pool = capturing ? g_stage_pinned : g_stage;
+        pool.push_back(c);
+        slot = &pool.back();
+    }
+    memcpy(slot->p, host, bytes);
+    if ((e = hipMemcpyAsync(dev, slot->p, bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(e, "tap upload");
+    if (!capturing) {
+        if ((e = hipEventRecord(slot->ev, s)) != hipSuccess) return hip_fail(e, "staging event record");
+        slot->recorded = true;
+    }
+    return NDL_OK;
+}
+
 ndl_tensor dense_like(const ndl_tensor* t, void* data, int dtype)
 {
     ndl_tensor r = *t;
@@ -164,7 +223,8 @@ ndl_tensor dense_like(const ndl_tensor* t, void* data, int dtype)
 
 // -------------------------------------------------------- generic axis pass
 int run_axis_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& k,
-                  int accum, int rounding, int mode, const void* table, int EL, int ER, hipStream_t s)
+                  int accum, int rounding, int mode, const void* table, int EL, int ER, hipStream_t s,
+                  int W = 0, int L = 0)
 {
     int64_t n = numel(in);
     if (n == 0) return NDL_OK;
@@ -200,9 +260,25 @@ int run_axis_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const n
     a.table = table;
     a.EL = EL;
     a.ER = ER;
-    for (int t = 0; t < k.count; t++) {
-        a.taps_d[t] = k.taps[t];
-        a.taps_f[t] = (float)k.taps[t];
+    a.W = W;
+    a.L = L;
+    Scratch wide; // released (stream-ordered) after the launch
+    if (k.count <= kArgTaps) {
+        for (int t = 0; t < k.count; t++) {
+            a.taps_d[t] = k.taps[t];
+            a.taps_f[t] = (float)k.taps[t];
+        }
+    } else {
+        const size_t db = (size_t)k.count * sizeof(double), fb = (size_t)k.count * sizeof(float);
+        int rc = alloc_scratch(wide, db + fb, s);
+        if (rc) return rc;
+        std::vector<unsigned char> host(db + fb);
+        memcpy(host.data(), k.taps, db);
+        float* hf = (float*)(host.data() + db);
+        for (int t = 0; t < k.count; t++) hf[t] = (float)k.taps[t];
+        if ((rc = stage_upload(wide.p, host.data(), db + fb, s))) return rc;
+        a.wide_d = (const double*)wide.p;
+        a.wide_f = (const float*)((unsigned char*)wide.p + db);
     }
     hipError_t e = launch_axis_pass(a, in->dtype, out->dtype, accum, s);
     if (e != hipSuccess) return hip_fail(e, "axis pass launch");
@@ -727,9 +803,115 @@ int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const nd
     return -1;
 }
 
+// uint8 images with small non-negative integer taps on at most two axes:
+// the byte kernel (exact 32-bit integer sums, bit-identical to the
+// reference's FP64 sums and cast).  Returns -1 when it does not apply.
+int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+              const ndl_axis_kernel* k, hipStream_t s)
+{
+    if (src->dtype != NDL_U8 || dst->dtype != NDL_U8 || naxes < 1 || naxes > 2) return -1;
+    if (!env_int("NDL_BYTES", 1)) return -1;
+    const int nd = src->ndim;
+    int xd = -1;
+    for (int d = 0; d < nd; d++)
+        if (src->stride[d] == 1 && dst->stride[d] == 1 && src->extent[d] > 1) { xd = d; break; }
+    if (xd < 0) return -1;
+    int conv[NDL_MAX_DIM];
+    for (int d = 0; d < nd; d++) conv[d] = -1;
+    for (int i = 0; i < naxes; i++) {
+        if (conv[axes[i]] >= 0) return -1;
+        conv[axes[i]] = i;
+        for (int t = 0; t < k[i].count; t++) {
+            const double w = k[i].taps[t];
+            if (!(w >= 0.0 && w <= 255.0) || w != std::floor(w)) return -1;
+        }
+    }
+    int yd = -1, others = 0;
+    for (int i = 0; i < naxes; i++)
+        if (axes[i] != xd) { yd = axes[i]; others++; }
+    if (others > 1) return -1; // the byte kernel convolves x and one more axis
+    if (yd < 0) { // only x convolved: march down the untouched dimension of smallest stride
+        for (int d = 0; d < nd; d++)
+            if (d != xd && src->extent[d] > 1 && (yd < 0 || std::llabs(src->stride[d]) < std::llabs(src->stride[yd])))
+                yd = d;
+    }
+    const int64_t X = src->extent[xd];
+    if (X % 4 != 0 || X > (1 << 30)) return -1;
+    const int64_t Y = yd >= 0 ? src->extent[yd] : 1;
+    const int64_t sy = yd >= 0 ? src->stride[yd] : 0, dy = yd >= 0 ? dst->stride[yd] : 0;
+    if (sy < 0 || dy < 0 || sy % 4 || dy % 4 || ((uintptr_t)src->data & 3) || ((uintptr_t)dst->data & 3)) return -1;
+    if ((Y - 1) * std::max(sy, dy) + X >= 0x7fffffffLL) return -1;
+    const int kx = conv[xd] >= 0 ? k[conv[xd]].count : 1;
+    const int ky = (yd >= 0 && conv[yd] >= 0) ? k[conv[yd]].count : 1;
+    if (!bytes_2d_supported(kx, ky)) return -1;
+    ByteArgs a;
+    memset(&a, 0, sizeof a);
+    a.src = (const uint8_t*)src->data;
+    a.dst = (uint8_t*)dst->data;
+    a.X = (int32_t)X;
+    a.Y = (int32_t)Y;
+    a.sy = (int32_t)sy;
+    a.dy = (int32_t)dy;
+    if (conv[xd] >= 0) {
+        const ndl_axis_kernel& ak = k[conv[xd]];
+        a.cx = ak.center;
+        a.bx = ak.border;
+        for (int t = 0; t < ak.count; t++) a.wx[t / 4] |= (uint32_t)ak.taps[t] << (8 * (t % 4));
+    } else {
+        a.cx = 0;
+        a.bx = NDL_CLAMP;
+        a.wx[0] = 1u;
+    }
+    if (a.cx != kx / 2) return -1; // the kernel's x windows assume the default centre
+    if (yd >= 0 && conv[yd] >= 0) {
+        const ndl_axis_kernel& ak = k[conv[yd]];
+        a.cy = ak.center;
+        a.by = ak.border;
+        for (int t = 0; t < ak.count; t++) a.wy[t] = (uint32_t)ak.taps[t];
+    } else {
+        a.cy = 0;
+        a.by = NDL_CLAMP;
+        a.wy[0] = 1u;
+    }
+    int64_t batch = 1;
+    for (int d = 0; d < nd; d++) {
+        if (d == xd || d == yd || src->extent[d] == 1) continue;
+        if (src->stride[d] % 4 || dst->stride[d] % 4) return -1;
+        a.bext[a.nb].init((uint32_t)src->extent[d]);
+        a.sb[a.nb] = src->stride[d];
+        a.db[a.nb] = dst->stride[d];
+        batch *= src->extent[d];
+        a.nb++;
+    }
+    a.strips = (int32_t)((X + 255) / 256);
+    // about eight waves per CU; chunks of at least 16 rows (the y halo is
+    // ky - 1 rows per chunk)
+    const int64_t cols = (int64_t)a.strips * batch;
+    const int64_t want = (int64_t)device_cus() * env_int("NDL_BYTES_WAVES_PER_CU", 8);
+    int64_t rows = std::max<int64_t>(env_int("NDL_BYTES_ROWS", 0) > 0 ? env_int("NDL_BYTES_ROWS", 0) : 16,
+                                     (Y * cols + want - 1) / want);
+    if (env_int("NDL_BYTES_ROWS", 0) > 0) rows = env_int("NDL_BYTES_ROWS", 0);
+    rows = std::min<int64_t>(rows, Y);
+    a.rows = (int32_t)rows;
+    a.chunks = (int32_t)((Y + rows - 1) / rows);
+    const int64_t waves = cols * a.chunks;
+    if (waves > 0x7fffffffLL) return -1;
+    a.waves = (uint32_t)waves;
+    a.fstrips.init((uint32_t)a.strips);
+    a.fchunks.init((uint32_t)a.chunks);
+    hipError_t e = launch_bytes_2d(a, kx, ky, s);
+    if (e == hipErrorInvalidValue) return -1;
+    if (e != hipSuccess) return hip_fail(e, "byte kernel launch");
+    return NDL_OK;
+}
+
 int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                  const ndl_axis_kernel* k, int32_t accum, int32_t rounding, hipStream_t s)
 {
+    if (src->dtype == NDL_U8 && dst->dtype == NDL_U8) {
+        const int rc = try_bytes(src, dst, naxes, axes, k, s);
+        if (rc != -1) return rc;
+    }
     if (naxes == 1) {
         const int rc = try_one_axis(src, dst, axes[0], k[0], accum, s);
         if (rc != -1) return rc;
@@ -771,7 +953,8 @@ struct AdjAxis {
     ndl_axis_kernel k;
     bool table;
     int EL = 0, ER = 0;       // edge rows at the start / end
-    size_t nw = 0;            // (EL + ER) * K weights
+    int W = 0, L = 0;         // edge-row window: width and left reach
+    size_t nw = 0;            // (EL + ER) * W weights
 };
 
 bool symmetric(const ndl_axis_kernel& k)
@@ -793,12 +976,18 @@ void build_adjoint_axis(const ndl_axis_kernel& k, int64_t m, AdjAxis& out)
     out.k.border = k.border;
     out.table = !(k.border == NDL_WRAP || (k.border == NDL_REFLECT && symmetric(k)));
     if (!out.table) return;
-    // row i of A^T in window form: out[i] = sum_s V[i][s] * g[i + s - c'],
-    // V[i][s] = sum_t w[t] [B(j + t - c) == i], j = i + s - c'.  Rows at
-    // least K - 1 from both ends equal the flipped band.
-    out.EL = (int)std::min<int64_t>(K, m);
-    out.ER = (int)std::min<int64_t>(K, m - out.EL);
-    out.nw = (size_t)(out.EL + out.ER) * K;
+    // row i of A^T in window form: out[i] = sum_s V[i][s] * g[i + s - L],
+    // V[i][s] = sum_t w[t] [B(j + t - c) == i], j = i + s - L.  The band's
+    // own reach is c' = K-1-c to the left and c to the right; a reflect fold
+    // reaches c-1 to the left of its row (from j >= 0) and c'-1 to the right
+    // (from j < M), so the window spans both.  Rows farther than the window
+    // from both ends equal the flipped band.
+    const int cp = K - 1 - c;
+    out.L = std::max(cp, c - 1);
+    out.W = out.L + std::max(c, cp - 1) + 1;
+    out.EL = (int)std::min<int64_t>(out.W, m);
+    out.ER = (int)std::min<int64_t>(out.W, m - out.EL);
+    out.nw = (size_t)(out.EL + out.ER) * out.W;
 }
 
 // edge weight rows of the table axes `which` into dst[q] (float or double
@@ -807,6 +996,7 @@ int fill_edge_tables(const ndl_tensor* g, const int32_t* axes, const ndl_axis_ke
                      const std::vector<AdjAxis>& adj, const std::vector<int>& which, const std::vector<void*>& dst,
                      int dt, hipStream_t s)
 {
+    std::deque<Scratch> wide; // device taps of axes wider than kArgTaps, freed after the builds
     for (size_t g0 = 0; g0 < which.size(); g0 += 3) {
         EdgeTables et;
         memset(&et, 0, sizeof et);
@@ -817,8 +1007,18 @@ int fill_edge_tables(const ndl_tensor* g, const int32_t* axes, const ndl_axis_ke
             et.border[e] = k[i].border;
             et.EL[e] = adj[i].EL;
             et.ER[e] = adj[i].ER;
+            et.W[e] = adj[i].W;
+            et.L[e] = adj[i].L;
             et.M[e] = g->extent[axes[i]];
-            for (int t = 0; t < k[i].count; t++) et.taps[e][t] = k[i].taps[t];
+            if (k[i].count <= kArgTaps) {
+                for (int t = 0; t < k[i].count; t++) et.taps[e][t] = k[i].taps[t];
+            } else {
+                wide.emplace_back();
+                int rc = alloc_scratch(wide.back(), (size_t)k[i].count * sizeof(double), s);
+                if (rc) return rc;
+                if ((rc = stage_upload(wide.back().p, k[i].taps, (size_t)k[i].count * sizeof(double), s))) return rc;
+                et.wide[e] = (const double*)wide.back().p;
+            }
             if (dt == NDL_F64) et.outd[e] = (double*)dst[q];
             else et.outf[e] = (float*)dst[q];
         }
@@ -867,6 +1067,9 @@ int adjoint_fused_pass(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes,
     if (!plan_fused(g, gx, naxes, ap.rax.data(), ap.rk.data(), accum, fa, cfg, int_acc, 0, ap.any_table, rd)) return 0;
     std::vector<int> which, role;
     size_t nfl = 0;
+    // the fused kernel's edge rows span the band's own window
+    for (int i = 0; i < naxes; i++)
+        if (ap.adj[i].table && ap.adj[i].W != k[i].count) return 0;
     for (int r = 0; r < 3; r++)
         for (int i = 0; rd[r] >= 0 && i < naxes; i++)
             if (axes[i] == rd[r] && ap.adj[i].table) {
@@ -952,7 +1155,8 @@ int adjoint_input_impl(const ndl_tensor* g, const ndl_tensor* gx, int32_t naxes,
         const ndl_tensor* out = last ? gx : &mid[i & 1];
         const AdjAxis& a = adj[src_i];
         rc = run_axis_pass(cur, out, axes[src_i], a.k, accum, NDL_ROUND_TRUNC, a.table ? PASS_TABLE : PASS_GATHER,
-                           a.table ? (const void*)((unsigned char*)tabs.p + tab_off[src_i]) : nullptr, a.EL, a.ER, s);
+                           a.table ? (const void*)((unsigned char*)tabs.p + tab_off[src_i]) : nullptr, a.EL, a.ER, s,
+                           a.W, a.L);
         if (rc) return rc;
         cur = out;
     }
@@ -966,15 +1170,22 @@ int adjoint_taps_impl(const ndl_tensor* x, const ndl_tensor* g, int32_t naxes, c
     const size_t acc_sz = dtype_size(acc_dt);
     int64_t n = numel(x);
     if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    // U and the per-block partial sums are needed only by axes that take the
+    // materialised route; the fused route keeps U in the tile
     Scratch ubuf, part;
-    int rc = alloc_scratch(ubuf, (size_t)n * acc_sz, s);
-    if (rc) return rc;
-    ndl_tensor u = dense_like(x, ubuf.p, acc_dt);
+    int rc = NDL_OK;
+    ndl_tensor u{};
     // about eight workgroups per CU on large volumes, 2048 voxels per workgroup at least
     const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>(8 * device_cus(), (n + 2047) / 2048));
     int maxk = 0;
     for (int i = 0; i < naxes; i++) maxk = std::max(maxk, k[i].count);
-    if ((rc = alloc_scratch(part, (size_t)nblocks * maxk * sizeof(double), s))) return rc;
+    auto need_u = [&]() -> int {
+        if (ubuf.p) return NDL_OK;
+        int r = alloc_scratch(ubuf, (size_t)n * acc_sz, s);
+        if (r) return r;
+        u = dense_like(x, ubuf.p, acc_dt);
+        return alloc_scratch(part, (size_t)nblocks * maxk * sizeof(double), s);
+    };
     static const double one = 1.0;
     int off = 0;
     for (int a = 0; a < naxes; a++) {
@@ -1008,6 +1219,7 @@ int adjoint_taps_impl(const ndl_tensor* x, const ndl_tensor* g, int32_t naxes, c
                 if (e != hipErrorInvalidValue) return hip_fail(e, "fused tap-gradient launch");
             }
         }
+        if ((rc = need_u())) return rc;
         // FP32 accumulation: the fused kernel when it applies (forward_impl
         // falls back to the chain); exact mode: the FP64 chain
         rc = accum == NDL_ACCUM_F32
@@ -1119,12 +1331,7 @@ int prepare_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kerne
     std::vector<char> host(voff + vb + 1);
     if (ob) memcpy(host.data(), offs.data(), ob);
     if (vb) memcpy(host.data() + voff, vals.data(), vb);
-    if (voff + vb) {
-        hipError_t e = hipMemcpyAsync(dev, host.data(), voff + vb, hipMemcpyHostToDevice, s);
-        // the host copy dies with this call: make sure the transfer has read it
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "kernel tap upload");
-    }
+    if ((rc = stage_upload(dev, host.data(), voff + vb, s))) return rc;
     a.in = src->data;
     a.out = dst->data;
     a.nd = nd;
@@ -1255,7 +1462,17 @@ int run_morph_pass(const ndl_tensor* in, const ndl_tensor* out, int axis, const 
     a.center = k.center;
     a.border = k.border;
     a.dilate = dilate;
-    for (int t = 0; t < k.count; t++) a.mask[t] = k.taps[t] != 0.0;
+    Scratch wide; // released (stream-ordered) after the launch
+    if (k.count <= kArgTaps) {
+        for (int t = 0; t < k.count; t++) a.mask[t] = k.taps[t] != 0.0;
+    } else {
+        std::vector<uint8_t> m(k.count);
+        for (int t = 0; t < k.count; t++) m[t] = k.taps[t] != 0.0;
+        int rc = alloc_scratch(wide, m.size(), s);
+        if (rc) return rc;
+        if ((rc = stage_upload(wide.p, m.data(), m.size(), s))) return rc;
+        a.wide_mask = (const uint8_t*)wide.p;
+    }
     hipError_t e = launch_morph_pass(a, in->dtype, out->dtype, s);
     if (e != hipSuccess) return hip_fail(e, "morphology pass launch");
     return NDL_OK;
@@ -1488,9 +1705,7 @@ NDL_API int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels,
     if ((rc = alloc_scratch(tb, (size_t)nlevels * NDL_MAX_TAPS * sizeof(double), s))) return rc;
     std::vector<double> host((size_t)nlevels * NDL_MAX_TAPS, 0.0);
     for (int l = 0; l < nlevels; l++) std::copy(taps[l].begin(), taps[l].end(), host.begin() + (size_t)l * NDL_MAX_TAPS);
-    hipError_t e = hipMemcpyAsync(tb.p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s); // the host copy dies with this call
-    if (e != hipSuccess) return hip_fail(e, "scale taps upload");
+    if ((rc = stage_upload(tb.p, host.data(), host.size() * sizeof(double), s))) return rc;
     MultiAxisArgs ma;
     memset(&ma, 0, sizeof ma);
     AxisPassArgs& p = ma.p;
@@ -1518,7 +1733,7 @@ NDL_API int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels,
         ma.K[l] = k[l][0].count;
         ma.center[l] = k[l][0].center;
     }
-    e = launch_axis_pass_multi(ma, src->dtype, s);
+    hipError_t e = launch_axis_pass_multi(ma, src->dtype, s);
     if (e != hipSuccess) return hip_fail(e, "scale-stack pass launch");
     for (int l = 0; l < nlevels; l++)
         if ((rc = run_generic_chain(&mid[l], &levels[l], nd - 1, axes + 1, k[l].data() + 1, accum, rounding, s)))

@@ -40,18 +40,18 @@ hipError_t edge_dispatch(const FusedArgs* a, const FusedConfig& cfg, int dt, int
 __global__ void edge_table_kernel(const EdgeTables e)
 {
     const int ax = blockIdx.y;
-    const int K = e.K[ax], c = e.c[ax], cp = K - 1 - c;
+    const int K = e.K[ax], c = e.c[ax], W = e.W[ax], L = e.L[ax];
     const int64_t M = e.M[ax];
     const int EL = e.EL[ax], ER = e.ER[ax];
-    const int n = (EL + ER) * K;
+    const int n = (EL + ER) * W;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
-        const int r = idx / K, s = idx - r * K;
+        const int r = idx / W, s = idx - r * W;
         const int64_t i = r < EL ? r : M - ER + (r - EL);
-        const int64_t j = i + s - cp;
+        const int64_t j = i + s - L;
         double v = 0.0;
         if (j >= 0 && j < M)
             for (int t = 0; t < K; t++)
-                if (border_map(e.border[ax], M, j + t - c) == i) v += e.taps[ax][t];
+                if (border_map(e.border[ax], M, j + t - c) == i) v += e.wide[ax] ? e.wide[ax][t] : e.taps[ax][t];
         if (e.outf[ax]) e.outf[ax][idx] = (float)v;
         if (e.outd[ax]) e.outd[ax][idx] = v;
     }
@@ -78,7 +78,7 @@ int fused_edge_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, siz
 hipError_t launch_edge_tables(const EdgeTables& e, hipStream_t s)
 {
     int rows = 0;
-    for (int a = 0; a < e.naxes; a++) rows = std::max(rows, (e.EL[a] + e.ER[a]) * e.K[a]);
+    for (int a = 0; a < e.naxes; a++) rows = std::max(rows, (e.EL[a] + e.ER[a]) * e.W[a]);
     if (e.naxes <= 0 || rows == 0) return hipSuccess;
     hipLaunchKernelGGL(edge_table_kernel, dim3((rows + 255) / 256, e.naxes), dim3(256), 0, s, e);
     return hipGetLastError();

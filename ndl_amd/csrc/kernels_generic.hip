@@ -24,8 +24,14 @@ template <> __device__ __forceinline__ float mac<float>(float acc, float v, floa
 }
 
 template <class Acc> __device__ __forceinline__ Acc tap_of(const AxisPassArgs& a, int t);
-template <> __device__ __forceinline__ double tap_of<double>(const AxisPassArgs& a, int t) { return a.taps_d[t]; }
-template <> __device__ __forceinline__ float tap_of<float>(const AxisPassArgs& a, int t) { return a.taps_f[t]; }
+template <> __device__ __forceinline__ double tap_of<double>(const AxisPassArgs& a, int t)
+{
+    return a.wide_d ? a.wide_d[t] : a.taps_d[t];
+}
+template <> __device__ __forceinline__ float tap_of<float>(const AxisPassArgs& a, int t)
+{
+    return a.wide_f ? a.wide_f[t] : a.taps_f[t];
+}
 
 template <class T, class Acc> __device__ __forceinline__ void store_acc(T* p, Acc v, int rounding)
 {
@@ -61,13 +67,22 @@ __global__ __launch_bounds__(256) void axis_pass_kernel(const AxisPassArgs a)
         }
     } else {
         const Acc* row = nullptr;
-        if (j < a.EL) row = (const Acc*)a.table + j * a.K;
-        else if (j >= a.m - a.ER) row = (const Acc*)a.table + (a.EL + (j - (a.m - a.ER))) * a.K;
-        for (int t = 0; t < a.K; t++) {
-            Acc w = row ? row[t] : tap_of<Acc>(a, t);
-            int64_t x = j + t - a.center;
-            if (x < 0 || x >= a.m || w == (Acc)0) continue; // zero outside the axis
-            total = mac<Acc>(total, load_acc<Acc>(in + x * a.in_as), w);
+        if (j < a.EL) row = (const Acc*)a.table + j * a.W;
+        else if (j >= a.m - a.ER) row = (const Acc*)a.table + (a.EL + (j - (a.m - a.ER))) * a.W;
+        if (row) {
+            for (int s = 0; s < a.W; s++) {
+                const Acc w = row[s];
+                const int64_t x = j + s - a.L;
+                if (x < 0 || x >= a.m || w == (Acc)0) continue; // zero outside the axis
+                total = mac<Acc>(total, load_acc<Acc>(in + x * a.in_as), w);
+            }
+        } else {
+            for (int t = 0; t < a.K; t++) {
+                const Acc w = tap_of<Acc>(a, t);
+                const int64_t x = j + t - a.center;
+                if (x < 0 || x >= a.m || w == (Acc)0) continue;
+                total = mac<Acc>(total, load_acc<Acc>(in + x * a.in_as), w);
+            }
         }
     }
     store_acc<TOut, Acc>((TOut*)a.out + ooff, total, a.rounding);
@@ -204,7 +219,7 @@ __global__ __launch_bounds__(256) void morph_pass_kernel(const MorphPassArgs a)
     TIn best{};
     Key bk{};
     for (int t = 0; t < a.K; t++) {
-        if (!a.mask[t]) continue; // uniform
+        if (!(a.wide_mask ? a.wide_mask[t] : a.mask[t])) continue; // uniform
         int64_t x = j + t - a.center;
         if (x < 0 || x >= a.m) x = border_map(a.border, a.m, x);
         const TIn v = in[x * a.in_as];
@@ -300,8 +315,8 @@ hipError_t launch_morph_pass(const MorphPassArgs& a, int in_dt, int out_dt, hipS
 {
     if (a.total == 0) return hipSuccess;
     // every tap included (box elements): compile-time tap counts
-    bool all = a.K >= 1;
-    for (int t = 0; t < a.K; t++) all = all && a.mask[t];
+    bool all = a.K >= 1 && !a.wide_mask;
+    for (int t = 0; t < a.K && !a.wide_mask; t++) all = all && a.mask[t];
     // (same-type passes and the common box sizes; the rest take the
     // run-time-tap kernel below)
     if (all && in_dt == out_dt && (a.K == 3 || a.K == 5 || a.K == 7)) {

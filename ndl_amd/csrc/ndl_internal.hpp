@@ -5,9 +5,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "ndl_common.hpp"
 
 namespace ndl {
+
+// Taps carried inline in kernel arguments; wider axes (up to NDL_MAX_TAPS)
+// pass a device array instead (the generic per-axis and morphology passes).
+constexpr int kArgTaps = 64;
 
 // ------------------------------------------------------- generic axis pass
 // One output per thread, outputs enumerated in `order` (dims sorted by
@@ -28,9 +35,12 @@ struct AxisPassArgs {
     uint32_t total;  // outputs (< 2^32)
     int32_t K, center, border, rounding, mode;
     int32_t EL, ER;       // PASS_TABLE: edge rows with their own weights
-    const void* table;    // PASS_TABLE: (EL+ER) rows of K weights (Acc type), device
-    double taps_d[NDL_MAX_TAPS];
-    float taps_f[NDL_MAX_TAPS];
+    int32_t W, L;         // PASS_TABLE: row width and left reach (row entry s reads position j + s - L)
+    const void* table;    // PASS_TABLE: (EL+ER) rows of W weights (Acc type), device
+    double taps_d[kArgTaps];
+    float taps_f[kArgTaps];
+    const double* wide_d;   // K > kArgTaps: the taps on the device (else null)
+    const float* wide_f;
 };
 
 // accum: NDL_ACCUM_REF (FP64, separately rounded) or NDL_ACCUM_F32 (FP32 FMA)
@@ -74,7 +84,8 @@ struct MorphPassArgs {
     uint32_t total;
     int32_t K, center, border;
     int32_t dilate;               // 0: minimum, 1: maximum
-    uint8_t mask[NDL_MAX_TAPS];   // included taps of this axis
+    uint8_t mask[kArgTaps];       // included taps of this axis (K <= kArgTaps)
+    const uint8_t* wide_mask;     // K > kArgTaps: the mask on the device
     const void* corner;           // original source for the NaN rule, or null
     int64_t cst[NDL_MAX_DIM];     // its strides (iteration order)
     int32_t coff[NDL_MAX_DIM];    // first included tap - centre per dimension (0: none)
@@ -217,13 +228,17 @@ int fused_grad_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, siz
 // Edge weight rows of transposed axis operators, built on the device:
 // for axis a of extent M with taps w (count K, centre c) and border B,
 // row r covers output i = r (r < EL) or M - ER + (r - EL), and
-// V[r][s] = sum_t w[t] [B(j + t - c) == i] with j = i + s - (K-1-c)
-// (0 when j is outside [0, M)); sums in FP64, t ascending.
+// V[r][s] = sum_t w[t] [B(j + t - c) == i] with j = i + s - L, s < W
+// (0 when j is outside [0, M)); sums in FP64, t ascending.  The window
+// (left reach L, width W) holds every j that folds onto i: L = K-1-c and
+// W = K for kernels centred within one tap, wider otherwise (a reflect fold
+// of an off-centre kernel lands beyond the band's reach).
 struct EdgeTables {
     int32_t naxes;
-    int32_t K[3], c[3], border[3], EL[3], ER[3];
+    int32_t K[3], c[3], border[3], EL[3], ER[3], W[3], L[3];
     int64_t M[3];
-    double taps[3][NDL_MAX_TAPS];
+    double taps[3][kArgTaps];
+    const double* wide[3];   // K > kArgTaps: that axis's taps on the device
     float* outf[3];   // float rows (or null)
     double* outd[3];  // double rows (or null)
 };
@@ -245,8 +260,8 @@ struct LineArgs {
     FastDiv bext[NDL_MAX_DIM];
     int64_t sb[NDL_MAX_DIM];
     int64_t db[NDL_MAX_DIM];
-    float w[NDL_MAX_TAPS];   // NDL_ACCUM_F32
-    double wd[NDL_MAX_TAPS]; // NDL_ACCUM_REF
+    float w[kArgTaps];       // NDL_ACCUM_F32
+    double wd[kArgTaps];     // NDL_ACCUM_REF
     // last erode / dilate pass of a float image: the original source, whose
     // value at the structuring element's corner (per dimension: first-tap
     // offset, border) replaces the result when it is NaN; null otherwise
@@ -283,13 +298,34 @@ struct RowArgs {
     FastDiv bext[NDL_MAX_DIM];
     int64_t sb[NDL_MAX_DIM];
     int64_t db[NDL_MAX_DIM];
-    float w[NDL_MAX_TAPS];   // NDL_ACCUM_F32
-    double wd[NDL_MAX_TAPS]; // NDL_ACCUM_REF
+    float w[kArgTaps];       // NDL_ACCUM_F32
+    double wd[kArgTaps];     // NDL_ACCUM_REF
 };
 // key.d is unused for rows
 hipError_t launch_row_pass(const RowArgs& a, const LineKey& key, uint32_t blocks, hipStream_t s);
 bool row_pass_supported(const LineKey& key);
 int row_pass_blocks_per_cu(const LineKey& key);
+// Two-axis passes over uint8 images with non-negative integer taps (at most
+// 9 per axis, x reach <= 4), four pixels per lane (kernels_bytes.hip).
+// x is the unit-stride dimension (X a multiple of 4, dword-aligned rows);
+// byte strides.  Each wave owns a 256-pixel strip of `rows` output rows.
+struct ByteArgs {
+    const uint8_t* src;
+    uint8_t* dst;
+    int32_t X, Y;            // x and y extents
+    int32_t sy, dy;          // y strides (bytes, positive)
+    int32_t cx, bx, cy, by;  // centres and borders
+    int32_t rows, strips, chunks;
+    uint32_t waves;          // strips * chunks * batch
+    FastDiv fstrips, fchunks;
+    int32_t nb;              // batch dimensions
+    FastDiv bext[NDL_MAX_DIM];
+    int64_t sb[NDL_MAX_DIM], db[NDL_MAX_DIM];
+    uint32_t wx[3];          // x taps, four 8-bit taps per word, window order, zero-padded
+    uint32_t wy[9];          // y taps
+};
+bool bytes_2d_supported(int kx, int ky);
+hipError_t launch_bytes_2d(const ByteArgs& a, int kx, int ky, hipStream_t s);
 // resident workgroups per CU for this instantiation (0 if unknown)
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);
 // whether an instantiation exists (queue: the persistent work-queue kernel)

@@ -720,3 +720,133 @@ def test_u8_integer_taps_fused_bit_exact(taps7):
         taps = [taps7] * len(axes)
         want = oracle.sepconv(x, axes, taps, ["wrap"] * len(axes))
         np.testing.assert_array_equal(gpu_sep(x, axes, taps, ["wrap"] * len(axes), REF), want)
+
+
+# ------------------------------------------- uint8 packed-byte kernel (x, y)
+# planar uint8 images whose x extent is a multiple of 4 with small integer
+# taps in [0, 255] take the four-pixels-per-lane byte kernel; every other
+# case (non-default x centre, x extent not a multiple of 4, three convolved
+# axes) takes the float paths.  All of them are bit-exact.
+BYTE_CASES = [
+    # (numpy shape, ndl axes, x taps, y taps, x border, y border)
+    ((2, 37, 260), [0, 1], np.ones(3), np.ones(3), "clamp", "reflect"),
+    ((3, 64, 1024), [0, 1], np.array([1, 4, 6, 4, 1.0]), np.array([1, 4, 6, 4, 1.0]), "reflect", "wrap"),
+    ((1, 23, 516), [0, 1], np.array([255, 0, 17, 3, 200, 9, 1.0]), np.array([7, 255, 1, 0, 99, 2, 5.0]),
+     "wrap", "clamp"),
+    ((2, 19, 132), [0, 1], np.arange(1, 10.0), np.arange(9, 0, -1.0), "reflect", "reflect"),
+    ((4, 8), [0, 1], np.ones(9), np.ones(9), "clamp", "wrap"),   # X = 8, Y = 4 < taps: repeated folds
+    ((3, 4), [0, 1], np.ones(9), np.ones(9), "reflect", "clamp"),  # minimal X (reach 4 = X)
+    ((5, 33, 400), [0], np.ones(7), None, "wrap", None),          # x only: march down y
+    ((5, 33, 400), [1], None, np.arange(1, 8.0), None, "reflect"),  # y only
+    ((6, 40, 96), [0, 2], np.ones(5), np.ones(5), "clamp", "clamp"),  # x and z: y becomes a batch axis
+]
+
+
+@pytest.mark.parametrize("case", range(len(BYTE_CASES)))
+def test_u8_byte_kernel_bit_exact(case):
+    shape, axes, tx, ty, bx, by = BYTE_CASES[case]
+    rng = np.random.default_rng(40 + case)
+    x = rng.integers(0, 256, size=shape, dtype=np.uint8)
+    taps, borders = [], []
+    for a in axes:
+        t, b = (tx, bx) if a == 0 else (ty, by)
+        taps.append(t)
+        borders.append(b)
+    say("u8 byte kernel", case, shape, axes)
+    want = oracle.sepconv(x, axes, taps, borders)
+    np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, REF), want)
+
+
+def test_u8_byte_kernel_views_and_fallbacks():
+    rng = np.random.default_rng(60)
+    x = rng.integers(0, 256, size=(3, 50, 208), dtype=np.uint8)
+    box = [np.ones(5), np.ones(5)]
+    # a row-strided view (y stride 2 rows), and a channel-reversed view
+    for view in (np.s_[:, ::2, :], np.s_[::-1, :, :]):
+        say("u8 byte kernel view", view)
+        want = oracle.sepconv(np.ascontiguousarray(x[view]), [0, 1], box, ["wrap", "clamp"])
+        np.testing.assert_array_equal(gpu_sep(x, [0, 1], box, ["wrap", "clamp"], REF, view=view), want)
+    # x centre off the middle, and an x extent that is not a multiple of 4
+    for xx, centers in ((x, [1, 2]), (x[:, :, :205].copy(), None)):
+        say("u8 byte kernel fallback", xx.shape, centers)
+        want = oracle.sepconv(xx, [0, 1], box, ["reflect", "reflect"], centers=centers)
+        np.testing.assert_array_equal(gpu_sep(xx, [0, 1], box, ["reflect", "reflect"], REF, centers=centers), want)
+
+
+# ------------------------------------------------------------- wide filters
+# gaussian_blur has no width cap (convolution.h:116-126): sigma 12 and 20
+# give 73 and 121 taps, beyond the 64 taps carried in kernel arguments; such
+# axes take the per-axis passes with their taps in a device array
+@pytest.mark.parametrize("sigma", [12.0, 20.0])
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+def test_wide_gaussian_blur_bit_exact(sigma, border):
+    rng = np.random.default_rng(int(sigma) * 7 + len(border))
+    x = rng.random((64, 70, 130)).astype(np.float32)  # reach 36 / 60 <= every extent
+    say("wide gaussian", sigma, border)
+    want = oracle.gaussian_blur(x, sigma, border)
+    got = ndl_amd.gaussian_blur(x, sigma, border, accum=REF)
+    np.testing.assert_array_equal(got, want)
+    taps = [workloads.gaussian_taps(sigma)] * 3
+    got32 = gpu_sep(x, [0, 1, 2], taps, [border] * 3, F32)
+    assert rel_err(got32, want) <= 1e-6
+    # integer image: std::round into uint8 (convolution.h:180-185)
+    xu = rng.integers(0, 256, size=(64, 130, 127), dtype=np.uint8)
+    np.testing.assert_array_equal(ndl_amd.gaussian_blur(xu, sigma, border), oracle.gaussian_blur(xu, sigma, border))
+
+
+@pytest.mark.parametrize("border", ["clamp", "wrap", "reflect"])
+def test_wide_one_axis_and_adjoint(border):
+    # an 81-tap asymmetric 1-D kernel (convolve with a one-axis kernel image)
+    # off centre, forward in both arithmetics and the input adjoint against
+    # the oracle's FP64 transpose (clamp / asymmetric reflect: edge weight
+    # rows built from the device tap array)
+    rng = np.random.default_rng(81)
+    x = rng.standard_normal((60, 55, 200))  # ndl extents 200, 55, 60: every reach (30 / 50) fits
+    taps = [rng.random(81) + 0.1]
+    for axis in (0, 2):
+        say("wide 81-tap axis", axis, border)
+        want = oracle.sepconv(x, [axis], taps, [border], centers=[30])
+        got = gpu_sep(x, [axis], taps, [border], REF, centers=[30])
+        np.testing.assert_array_equal(got, want)
+        g = rng.standard_normal(x.shape)
+        gw = oracle.adjoint_input(g, [axis], taps, [border], centers=[30])
+        d = ndl_amd.DeviceArray.from_numpy(g)
+        o = ndl_amd.DeviceArray(g.shape, np.float64)
+        ndl_amd.adjoint_input_dev(d, o, [axis], taps, [border], REF, centers=[30])
+        np.testing.assert_allclose(o.to_numpy(), gw, rtol=1e-12, atol=1e-12 * np.abs(gw).max())
+
+
+def test_wide_box_erode_matches_oracle():
+    # a 71 x 3 box: the erode pass along x takes its inclusion mask from a device array
+    rng = np.random.default_rng(71)
+    x = rng.integers(0, 256, size=(90, 160), dtype=np.uint8)
+    mask = np.ones((3, 71))
+    say("wide erode")
+    np.testing.assert_array_equal(ndl_amd.morph(x, mask, "erode", "reflect"), oracle.morph(x, mask, "erode", "reflect"))
+
+
+@pytest.mark.parametrize("accum", ["ref", "f32"])
+@pytest.mark.parametrize("kc", [(9, 2), (9, 6), (5, 0), (12, 3), (4, 2)])
+def test_adjoint_reflect_off_centre(kc, accum):
+    # reflect with a kernel centred more than one tap off its middle: a fold
+    # lands beyond the band's reach, so the edge weight rows need a wider
+    # window than K (regression: K-wide rows lost those terms); every axis,
+    # against the oracle's FP64 transpose
+    K, c = kc
+    rng = np.random.default_rng(K * 10 + c)
+    taps = [rng.random(K) + 0.1 for _ in range(3)]
+    g = rng.standard_normal((21, 26, 33))
+    say("off-centre reflect adjoint", K, c, accum)
+    want = oracle.adjoint_input(g, [0, 1, 2], taps, ["reflect"] * 3, centers=[c] * 3)
+    if accum == "ref":
+        d = ndl_amd.DeviceArray.from_numpy(g)
+        o = ndl_amd.DeviceArray(g.shape, np.float64)
+        ndl_amd.adjoint_input_dev(d, o, [0, 1, 2], taps, ["reflect"] * 3, REF, centers=[c] * 3)
+        np.testing.assert_allclose(o.to_numpy(), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    else:
+        g32 = g.astype(np.float32)
+        d = ndl_amd.DeviceArray.from_numpy(g32)
+        o = ndl_amd.DeviceArray(g.shape, np.float32)
+        ndl_amd.adjoint_input_dev(d, o, [0, 1, 2], taps, ["reflect"] * 3, F32, centers=[c] * 3)
+        want32 = oracle.adjoint_input(g32.astype(np.float64), [0, 1, 2], taps, ["reflect"] * 3, centers=[c] * 3)
+        assert bound_err(o.to_numpy(), want32, g32, taps) <= 2e-6

@@ -68,7 +68,7 @@ def test_version_and_errors_without_gpu():
     assert lib.ndl_sepconv_forward(C.byref(t), C.byref(far), 1, ax, kb.arr, 0, 0, None) == abi.ERR_ARG
     kc = abi.Kernels([np.ones(3)], ["clamp"], centers=[3])
     assert lib.ndl_sepconv_forward(C.byref(t), C.byref(far), 1, ax, kc.arr, 0, 0, None) == abi.ERR_ARG
-    kt = abi.Kernels([np.ones(65)], ["clamp"])
+    kt = abi.Kernels([np.ones(abi.NDL_MAX_TAPS + 1)], ["clamp"])
     assert lib.ndl_sepconv_forward(C.byref(t), C.byref(far), 1, ax, kt.arr, 0, 0, None) == abi.ERR_ARG
     # adjoint on integer fields
     assert lib.ndl_sepconv_adjoint_input(C.byref(tu), C.byref(tu2), 1, ax, ks.arr, 0, None) == abi.ERR_DTYPE
