@@ -15,9 +15,11 @@ default the Infinity Cache is flushed with a 1 GiB read before every step
 roofline: algorithmic bytes per launch = (4 B read + 4 B write) per voxel of
 the fused kernel, divided by its mean launch time, against 8 TB/s.
 cpu_baseline: the reference's own convolve() chain (oracle/_ref, built from
-the reference sources) on one host core, full volume, fwd+adj counted as two
-forward applications (this reflect-Gaussian operator is self-adjoint); falls
-back to the C restatement when the reference build is absent.
+the reference sources) on one host core over a quarter slab of the volume,
+forward of x and input adjoint of w each timed (the reflect Gaussian is its
+own transpose), median of three samples; falls back to the C restatement when
+the reference build is absent.  An all-cores run of the restatement is
+reported beside it, labelled as a port.
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--warm] [--no-cpu]
 """
@@ -150,17 +152,22 @@ def main():
     launch_ms = float(np.mean(np.concatenate([f_ms, a_ms])))
     alg_bytes = 8 * nvox
     achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
-    traffic = None
+    # HBM bytes per launch: measured by two rocprofv3 --pmc runs of this bench
+    # (FETCH_SIZE, WRITE_SIZE, calibrated against kernels of known size) and
+    # committed with its commands; not measurable from inside this process
+    traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "profiles", f"traffic_gauss3d_{n}_f32.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+            tj = json.load(open(tfile))
+            traffic = tj.get("hbm_bytes_per_launch")
+            traffic_src = f"profiles/traffic_gauss3d_{n}_f32.json, from {tj.get('source', 'rocprofv3 --pmc')}"
         except (OSError, ValueError):
             traffic = None
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(x, axes, taps, borders, nvox)
+        cpu = cpu_baseline(x, w, axes, taps, borders)
 
     line = {
         "metric": METRIC,
@@ -186,12 +193,16 @@ def main():
             "cache": "cold (1 GiB read flush before each step)" if cold else "warm",
         },
         "roofline": {
+            # the roofline denominator is HBM; counters show the kernel itself is
+            # bound by VALU issue and LDS (see DESIGN.md's measurement notes)
             "bound": "hbm",
+            "measured_limiter": "valu+lds",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "fused_tile_kernel (fwd and adjoint launches)",
             "alg_bytes_per_launch": alg_bytes,
             "mean_launch_us": round(launch_ms * 1e3, 2),
@@ -205,27 +216,61 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(x, axes, taps, borders, nvox):
+def cpu_baseline(x, w, axes, taps, borders):
+    """The reference's CPU path on a bounded sample of the workload.
+
+    Sample: a 256 x 256 x 64 slab of the volume (one quarter; the per-voxel
+    cost of the serial loop does not depend on the slab's place), the same
+    taps and borders.  One sample = the forward of x plus the input adjoint
+    of w; this operator (reflect, symmetric taps) is its own transpose, so the
+    adjoint is timed as the reference convolve() chain applied to w.  Median
+    of three samples on one pinned core (the reference loop is serial,
+    convolution.h:64-70).  Also timed: the oracle's restatement on every host
+    core (bit-identical outputs), labelled separately as a port."""
     import oracle
-    y = np.empty_like(x)
+    nz = max(1, x.shape[0] // 4)
+    xs, ws = np.ascontiguousarray(x[:nz]), np.ascontiguousarray(w[:nz])
+    nvox = xs.size
+    y = np.empty_like(xs)
     try:
-        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        saved = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {sorted(saved)[0]})
     except (AttributeError, OSError):
-        pass
+        saved = None
     use_ref = oracle.ref_lib is not None
-    log(f"bench: CPU baseline ({'reference build' if use_ref else 'C restatement'}), one core")
-    t0 = time.perf_counter()
-    oracle.sepconv(x, axes, taps, borders, dst=y, use_ref=use_ref)
-    t = time.perf_counter() - t0
-    log(f"bench: CPU forward {t:.2f} s")
+    log(f"bench: CPU baseline ({'reference build' if use_ref else 'C restatement'}), one core, {xs.shape} slab x3")
+    samples = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.sepconv(xs, axes, taps, borders, dst=y, use_ref=use_ref)
+        t1 = time.perf_counter()
+        oracle.sepconv(ws, axes, taps, borders, dst=y, use_ref=use_ref)
+        samples.append((t1 - t0, time.perf_counter() - t1))
+    fwd = float(np.median([a for a, _ in samples]))
+    adj = float(np.median([b for _, b in samples]))
+    step = float(np.median([a + b for a, b in samples]))
+    if saved is not None:
+        try:
+            os.sched_setaffinity(0, saved)
+        except OSError:
+            pass
+    ncores = len(saved) if saved is not None else (os.cpu_count() or 1)
+    with oracle.threads(ncores):
+        t0 = time.perf_counter()
+        oracle.sepconv(xs, axes, taps, borders, dst=y)
+        oracle.sepconv(ws, axes, taps, borders, dst=y)
+        t_all = time.perf_counter() - t0
+    log(f"bench: CPU fwd {fwd:.2f} s, adj {adj:.2f} s (median of 3), all-cores port {t_all:.2f} s")
     return {
-        "value": round(nvox / (2 * t) / 1e9, 5),
+        "value": round(nvox / step / 1e9, 5),
         "unit": "Gvoxels/s",
         "cores": 1,
         "kind": "reference" if use_ref else "port",
-        "sample": (f"full {round(nvox ** (1 / 3))}^3 volume, one forward of the per-axis convolve() chain "
-                   f"({t:.2f} s), fwd+adj = 2 forwards (self-adjoint reflect Gaussian); host: {cpu_model()}, "
-                   f"nproc {os.cpu_count()}"),
+        "sample": (f"{xs.shape[2]}x{xs.shape[1]}x{xs.shape[0]} slab of the volume, forward of x + input adjoint "
+                   f"of w (the reflect Gaussian is self-adjoint: the reference convolve() chain on w), median of "
+                   f"3 samples: fwd {fwd:.3f} s, adj {adj:.3f} s; host: {cpu_model()}, nproc {os.cpu_count()}"),
+        "all_cores_port": {"value": round(nvox / t_all / 1e9, 5), "unit": "Gvoxels/s", "cores": ncores,
+                           "kind": "port", "note": "the oracle's C restatement on every core, not the reference"},
     }
 
 
