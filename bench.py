@@ -276,7 +276,12 @@ def cpu_baseline(x, w, axes, taps, borders):
 
 # --------------------------------------------------------------------- suite
 def _time_launches(hip, stream, fn, reps, flush=None):
-    """mean ms of fn() over reps, each bracketed by events (flush outside)."""
+    """mean ms of fn() over reps, each bracketed by events (flush outside).
+
+    `flush` runs before each rep's first event: the cache flush for cold
+    timings, or a spin kernel for warm ones. Either keeps the GPU busy
+    while the host enqueues the events and fn's launches, so the event pair
+    times GPU work only."""
     import ndl_amd
     ts = []
     for _ in range(reps):
@@ -302,11 +307,16 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
     lib = ndl_amd.lib()
     bu = C.CDLL(os.path.join(ROOT, "ndl_amd", "libndl_benchutil.so"))
     bu.ndl_bench_flush.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+    bu.ndl_bench_spin.argtypes = [C.c_double, C.c_void_p, C.c_void_p]
     dflush = ndl_amd.DeviceArray((1 << 30,), np.uint8)
     dsink = ndl_amd.DeviceArray((4096,), np.uint32)
 
     def flush():
         bu.ndl_bench_flush(dflush.ptr, 1 << 30, dsink.ptr, stream)
+
+    def spin():
+        # warm timings: 200 us of one busy wave ahead of the events
+        bu.ndl_bench_spin(200.0, dsink.ptr, stream)
 
     def fwd_fn(src, dst, axes, taps, borders, accum):
         ks = abi.Kernels(taps, borders)
@@ -328,8 +338,8 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
         d = ndl_amd.DeviceArray(x.shape, np.float32)
         f = fwd_fn(s, d, axes, taps, borders, abi.ACCUM_F32)
         f()
-        ms = _time_launches(hip, stream, f, reps)
-        emit({"config": "1: 2-D f32 512x512 sigma 2 clamp, forward", "us": round(ms * 1e3, 2),
+        ms = _time_launches(hip, stream, f, reps, spin)
+        emit({"config": "1: 2-D f32 512x512 sigma 2 clamp, forward, warm", "us": round(ms * 1e3, 2),
               "gvox_s": round(x.size / ms / 1e6, 2), "GBps_alg": round(8 * x.size / ms / 1e6, 1)})
     if 2 in configs:
         for inter in (False, True):
@@ -338,11 +348,13 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
             d = ndl_amd.DeviceArray(x.shape, np.uint8)
             f = fwd_fn(s, d, axes, taps, borders, abi.ACCUM_REF)
             f()
-            ms = _time_launches(hip, stream, f, reps)
+            ms = _time_launches(hip, stream, f, reps, spin)
+            ms_host = _time_launches(hip, stream, f, reps)
             ok = np.array_equal(d.to_numpy(), oracle.sepconv(x, axes, taps, borders))
-            emit({"config": f"2b: u8 2048x2048x3 box7 wrap ({'interleaved' if inter else 'planar'}), forward",
+            emit({"config": f"2b: u8 2048x2048x3 box7 wrap ({'interleaved' if inter else 'planar'}), forward, warm",
                   "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
-                  "GBps_alg": round(2 * x.size / ms / 1e6, 1), "bit_exact": bool(ok)})
+                  "GBps_alg": round(2 * x.size / ms / 1e6, 1), "bit_exact": bool(ok),
+                  "us_with_host_launch": round(ms_host * 1e3, 2)})
         # the 256^3 Gaussian's axes one at a time (convolve() with a 1-D
         # kernel image per axis), cold cache: x on the row kernel, y and z
         # on the line march; FP32 accumulation, then the reference's FP64
@@ -375,7 +387,7 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
         d = ndl_amd.DeviceArray(x.shape, np.float32)
         f = fwd_fn(s, d, axes, taps, borders, abi.ACCUM_F32)
         f()
-        ms = _time_launches(hip, stream, f, max(3, reps // 2))
+        ms = _time_launches(hip, stream, f, max(3, reps // 2), spin)
         emit({"config": "3: 4-D f32 1920x1080x3x32, x,y sigma 1 clamp + t binomial wrap, forward",
               "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
               "GBps_alg": round(8 * x.size / ms / 1e6, 1)})
@@ -405,7 +417,7 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
         ndl_amd.check(lib.ndl_plan_create(arr, len(bitems), abi.ACCUM_F32, C.byref(plan)))
         run_plan = lambda: ndl_amd.check(lib.ndl_plan_execute(plan, stream))  # noqa: E731
         run_plan()
-        ms_plan = _time_launches(hip, stream, run_plan, max(3, reps // 2))
+        ms_plan = _time_launches(hip, stream, run_plan, max(3, reps // 2), spin)
         singles = [fwd_fn(srcs[i], dsts[i], list(range(len(items[i][0]))), items[i][1],
                           [items[i][2]] * len(items[i][0]), abi.ACCUM_F32) for i in range(len(items))]
 
@@ -413,7 +425,7 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
             for f in singles:
                 f()
         run_singles()
-        ms_single = _time_launches(hip, stream, run_singles, max(3, reps // 2))
+        ms_single = _time_launches(hip, stream, run_singles, max(3, reps // 2), spin)
         ndl_amd.check(lib.ndl_plan_destroy(plan))
         emit({"config": "4: 256 uneven f32 instances (192 2-D, 64 3-D), one work-queue launch (hipGraph)",
               "voxels": nvox, "plan_us": round(ms_plan * 1e3, 1), "plan_gvox_s": round(nvox / ms_plan / 1e6, 2),

@@ -836,7 +836,7 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
                 yd = d;
     }
     const int64_t X = src->extent[xd];
-    if (X % 4 != 0 || X > (1 << 30)) return -1;
+    if (X % 4 != 0 || X < 256 || X > (1 << 30)) return -1; // whole 256-pixel strips (the last one may overlap)
     const int64_t Y = yd >= 0 ? src->extent[yd] : 1;
     const int64_t sy = yd >= 0 ? src->stride[yd] : 0, dy = yd >= 0 ? dst->stride[yd] : 0;
     if (sy < 0 || dy < 0 || sy % 4 || dy % 4 || ((uintptr_t)src->data & 3) || ((uintptr_t)dst->data & 3)) return -1;
@@ -884,13 +884,17 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
         a.nb++;
     }
     a.strips = (int32_t)((X + 255) / 256);
-    // about eight waves per CU; chunks of at least 16 rows (the y halo is
-    // ky - 1 rows per chunk)
+    // about 24 waves per CU, chunks of at least 8 rows (each chunk recomputes
+    // the x sums of ky - 1 halo rows).  A wave walks its rows serially, so
+    // more, shorter chunks finish sooner until the halo work dominates: on
+    // MI355X a 2048x2048x3 box 7 is fastest at 8-row chunks (6144 waves).
     const int64_t cols = (int64_t)a.strips * batch;
-    const int64_t want = (int64_t)device_cus() * env_int("NDL_BYTES_WAVES_PER_CU", 8);
-    int64_t rows = std::max<int64_t>(env_int("NDL_BYTES_ROWS", 0) > 0 ? env_int("NDL_BYTES_ROWS", 0) : 16,
-                                     (Y * cols + want - 1) / want);
+    const int64_t want = (int64_t)device_cus() * env_int("NDL_BYTES_WAVES_PER_CU", 24);
+    int64_t rows = std::max<int64_t>(8, (Y * cols + want - 1) / want);
     if (env_int("NDL_BYTES_ROWS", 0) > 0) rows = env_int("NDL_BYTES_ROWS", 0);
+    // whole unrolled blocks: rows + ky - 1 input rows a multiple of the block
+    const int nu = bytes_prefetch_rows(ky);
+    rows = (rows + ky - 1 + nu - 1) / nu * nu - (ky - 1);
     rows = std::min<int64_t>(rows, Y);
     a.rows = (int32_t)rows;
     a.chunks = (int32_t)((Y + rows - 1) / rows);

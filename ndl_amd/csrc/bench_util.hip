@@ -36,6 +36,29 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ a, 
     if (i < n16) b[i] = a[i];
 }
 
+// one wave that waits `ticks` of the 100 MHz real-time counter: queued
+// before a timed region it keeps the GPU busy while the host enqueues the
+// region's commands, so the events around them time the GPU work alone
+// (without it a short kernel's event pair also spans the host's launch
+// overhead).  Touches no memory but one word.
+__global__ __launch_bounds__(64) void spin_kernel(uint64_t ticks, uint32_t* sink)
+{
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    uint32_t n = 0;
+    while (__builtin_amdgcn_s_memrealtime() - t0 < ticks) {
+        __builtin_amdgcn_s_sleep(8);
+        n++;
+    }
+    if (threadIdx.x == 0 && n == 0xffffffffu) sink[0] = n;
+}
+
+extern "C" __attribute__((visibility("default"))) int ndl_bench_spin(double microseconds, void* sink, void* stream)
+{
+    const uint64_t ticks = (uint64_t)(microseconds * 100.0);
+    hipLaunchKernelGGL(spin_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ticks, (uint32_t*)sink);
+    return hipGetLastError() == hipSuccess ? 0 : -6;
+}
+
 extern "C" __attribute__((visibility("default"))) int ndl_bench_flush(const void* buf, size_t bytes, void* sink,
                                                                       void* stream)
 {

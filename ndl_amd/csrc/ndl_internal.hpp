@@ -307,7 +307,8 @@ bool row_pass_supported(const LineKey& key);
 int row_pass_blocks_per_cu(const LineKey& key);
 // Two-axis passes over uint8 images with non-negative integer taps (at most
 // 9 per axis, x reach <= 4), four pixels per lane (kernels_bytes.hip).
-// x is the unit-stride dimension (X a multiple of 4, dword-aligned rows);
+// x is the unit-stride dimension (X a multiple of 4 and at least 256,
+// dword-aligned rows);
 // byte strides.  Each wave owns a 256-pixel strip of `rows` output rows.
 struct ByteArgs {
     const uint8_t* src;
@@ -325,6 +326,9 @@ struct ByteArgs {
     uint32_t wy[9];          // y taps
 };
 bool bytes_2d_supported(int kx, int ky);
+// input rows a wave keeps in flight for ky y taps; a chunk whose rows + ky - 1
+// input rows are a multiple of it runs with no per-row checks
+constexpr int bytes_prefetch_rows(int ky) { return ky >= 5 ? 2 * ky : (ky == 3 ? 12 : 16); }
 hipError_t launch_bytes_2d(const ByteArgs& a, int kx, int ky, hipStream_t s);
 // resident workgroups per CU for this instantiation (0 if unknown)
 int fused_blocks_per_cu(const FusedConfig& cfg, int dt, int kx, int ky, size_t lds);

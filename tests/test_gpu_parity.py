@@ -723,22 +723,23 @@ def test_u8_integer_taps_fused_bit_exact(taps7):
 
 
 # ------------------------------------------- uint8 packed-byte kernel (x, y)
-# planar uint8 images whose x extent is a multiple of 4 with small integer
-# taps in [0, 255] take the four-pixels-per-lane byte kernel; every other
-# case (non-default x centre, x extent not a multiple of 4, three convolved
-# axes) takes the float paths.  All of them are bit-exact.
+# planar uint8 images whose x extent is a multiple of 4 and at least 256
+# with small integer taps in [0, 255] take the four-pixels-per-lane byte
+# kernel; every other case (non-default x centre, other x extents, three
+# convolved axes) takes the float paths.  All of them are bit-exact.
 BYTE_CASES = [
     # (numpy shape, ndl axes, x taps, y taps, x border, y border)
-    ((2, 37, 260), [0, 1], np.ones(3), np.ones(3), "clamp", "reflect"),
+    ((2, 37, 260), [0, 1], np.ones(3), np.ones(3), "clamp", "reflect"),   # last strip overlaps the first
     ((3, 64, 1024), [0, 1], np.array([1, 4, 6, 4, 1.0]), np.array([1, 4, 6, 4, 1.0]), "reflect", "wrap"),
     ((1, 23, 516), [0, 1], np.array([255, 0, 17, 3, 200, 9, 1.0]), np.array([7, 255, 1, 0, 99, 2, 5.0]),
      "wrap", "clamp"),
-    ((2, 19, 132), [0, 1], np.arange(1, 10.0), np.arange(9, 0, -1.0), "reflect", "reflect"),
-    ((4, 8), [0, 1], np.ones(9), np.ones(9), "clamp", "wrap"),   # X = 8, Y = 4 < taps: repeated folds
-    ((3, 4), [0, 1], np.ones(9), np.ones(9), "reflect", "clamp"),  # minimal X (reach 4 = X)
-    ((5, 33, 400), [0], np.ones(7), None, "wrap", None),          # x only: march down y
+    ((2, 19, 256), [0, 1], np.arange(1, 10.0), np.arange(9, 0, -1.0), "reflect", "reflect"),  # one strip, both ends fold
+    ((4, 256), [0, 1], np.ones(9), np.ones(9), "clamp", "wrap"),    # Y = 4 < taps: repeated row folds
+    ((3, 264), [0, 1], np.ones(9), np.ones(9), "wrap", "clamp"),
+    ((5, 33, 400), [0], np.ones(7), None, "wrap", None),            # x only: march down y
     ((5, 33, 400), [1], None, np.arange(1, 8.0), None, "reflect"),  # y only
-    ((6, 40, 96), [0, 2], np.ones(5), np.ones(5), "clamp", "clamp"),  # x and z: y becomes a batch axis
+    ((6, 40, 296), [0, 2], np.ones(5), np.ones(5), "clamp", "clamp"),  # x and z: y becomes a batch axis
+    ((300, 1100), [0, 1], np.ones(3), np.ones(9), "reflect", "wrap"),  # 14-row blocks and a partial last chunk
 ]
 
 
@@ -759,15 +760,16 @@ def test_u8_byte_kernel_bit_exact(case):
 
 def test_u8_byte_kernel_views_and_fallbacks():
     rng = np.random.default_rng(60)
-    x = rng.integers(0, 256, size=(3, 50, 208), dtype=np.uint8)
+    x = rng.integers(0, 256, size=(3, 50, 300), dtype=np.uint8)
     box = [np.ones(5), np.ones(5)]
     # a row-strided view (y stride 2 rows), and a channel-reversed view
     for view in (np.s_[:, ::2, :], np.s_[::-1, :, :]):
         say("u8 byte kernel view", view)
         want = oracle.sepconv(np.ascontiguousarray(x[view]), [0, 1], box, ["wrap", "clamp"])
         np.testing.assert_array_equal(gpu_sep(x, [0, 1], box, ["wrap", "clamp"], REF, view=view), want)
-    # x centre off the middle, and an x extent that is not a multiple of 4
-    for xx, centers in ((x, [1, 2]), (x[:, :, :205].copy(), None)):
+    # x centre off the middle, an x extent that is not a multiple of 4, and
+    # one below a strip (256)
+    for xx, centers in ((x, [1, 2]), (x[:, :, :205].copy(), None), (x[:, :, :132].copy(), None)):
         say("u8 byte kernel fallback", xx.shape, centers)
         want = oracle.sepconv(xx, [0, 1], box, ["reflect", "reflect"], centers=centers)
         np.testing.assert_array_equal(gpu_sep(xx, [0, 1], box, ["reflect", "reflect"], REF, centers=centers), want)
