@@ -222,8 +222,16 @@ int ndl_gaussian_taps(double sigma, double* taps, int32_t cap);
 
 /* ---- batched plans (no counterpart in the reference) --------------------
  * Many independent separable forwards of uneven extents, dimensions, taps
- * and borders, in one persistent work-queue launch.  The plan copies the
- * descriptors and taps, picks tile shapes, and owns its device work lists. */
+ * and borders.  The plan copies the descriptors and taps, groups the items
+ * by storage type, tile shape and ring period, and owns one device job list
+ * per group; each group runs as a persistent work-queue kernel, the groups
+ * side by side on up to four streams forked from and joined back onto the
+ * caller's stream (or, if that measured faster, one launch per item).
+ * Items the fused kernel cannot take run through the per-axis chain.
+ * The first ndl_plan_execute times every candidate schedule on the
+ * caller's stream and waits for those timings before it returns (the only
+ * call that blocks the host); it keeps the fastest and captures it into a
+ * HIP graph, and later calls only launch that graph, asynchronously. */
 typedef struct ndl_batch_item {
     ndl_tensor src;
     ndl_tensor dst;
