@@ -200,6 +200,20 @@ def test_scale_stack_and_dog_exact(shape):
         assert same_bits(d.to_numpy(), want[i + 1] - want[i])
 
 
+def test_scale_stack_wide_sigmas_exact():
+    # detect_keypoints' octaves reach large sigmas; 12.0 and 20.0 give 73 and
+    # 121 taps (past the 64 carried in kernel arguments), beside a narrow level
+    rng = np.random.default_rng(57)
+    x = rng.random((70, 130)) * 255.0
+    sig = [1.6, 12.0, 20.0]
+    say("scale stack, wide sigmas")
+    s = ndl_amd.DeviceArray.from_numpy(x)
+    levels = [ndl_amd.DeviceArray(x.shape, np.float64) for _ in sig]
+    ndl_amd.gaussian_scales_dev(s, levels, sig, "reflect")
+    for lv, sg in zip(levels, sig):
+        assert same_bits(lv.to_numpy(), oracle.gaussian_blur(x, sg, "reflect"))
+
+
 def test_scale_stack_uint8_and_fp32():
     rng = np.random.default_rng(56)
     x = rng.integers(0, 256, size=(64, 80)).astype(np.uint8)
