@@ -804,18 +804,15 @@ int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const nd
 }
 
 // uint8 images with small non-negative integer taps on at most two axes:
-// the byte kernel (exact 32-bit integer sums, bit-identical to the
-// reference's FP64 sums and cast).  Returns -1 when it does not apply.
+// the packed-byte kernels, planar or interleaved colour (exact integer sums
+// modulo 2^16, whose low byte is the reference's FP64 sum cast to uint8,
+// convolution.h:64-70).  Returns -1 when they do not apply.
 int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
               const ndl_axis_kernel* k, hipStream_t s)
 {
     if (src->dtype != NDL_U8 || dst->dtype != NDL_U8 || naxes < 1 || naxes > 2) return -1;
     if (!env_int("NDL_BYTES", 1)) return -1;
     const int nd = src->ndim;
-    int xd = -1;
-    for (int d = 0; d < nd; d++)
-        if (src->stride[d] == 1 && dst->stride[d] == 1 && src->extent[d] > 1) { xd = d; break; }
-    if (xd < 0) return -1;
     int conv[NDL_MAX_DIM];
     for (int d = 0; d < nd; d++) conv[d] = -1;
     for (int i = 0; i < naxes; i++) {
@@ -826,24 +823,58 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
             if (!(w >= 0.0 && w <= 255.0) || w != std::floor(w)) return -1;
         }
     }
+    // interleaved colour ({3, w, ...}: the channel dimension 0 untouched with
+    // unit stride, pixels along dimension 1 with stride 3, per_channel()'s
+    // layout, image/kernels.h:78-87): x convolved -> the colour kernel;
+    // otherwise channel and pixel merge into one byte axis of 3w bytes
+    bool rgb = false;
+    if (nd >= 2 && src->extent[0] == 3 && src->stride[0] == 1 && dst->stride[0] == 1 && src->stride[1] == 3 &&
+        dst->stride[1] == 3 && conv[0] < 0) {
+        if (conv[1] >= 0) {
+            rgb = true;
+        } else {
+            ndl_tensor ms = *src, md = *dst;
+            ms.extent[0] = md.extent[0] = 3 * src->extent[1];
+            for (int d = 1; d + 1 < nd; d++) {
+                ms.extent[d] = src->extent[d + 1];
+                ms.stride[d] = src->stride[d + 1];
+                md.extent[d] = dst->extent[d + 1];
+                md.stride[d] = dst->stride[d + 1];
+            }
+            ms.ndim = md.ndim = nd - 1;
+            int32_t max_[2];
+            for (int i = 0; i < naxes; i++) max_[i] = axes[i] - 1;
+            return try_bytes(&ms, &md, naxes, max_, k, s);
+        }
+    }
+    int xd = -1;
+    if (rgb) {
+        xd = 1;
+    } else {
+        for (int d = 0; d < nd; d++)
+            if (src->stride[d] == 1 && dst->stride[d] == 1 && src->extent[d] > 1) { xd = d; break; }
+    }
+    if (xd < 0) return -1;
     int yd = -1, others = 0;
     for (int i = 0; i < naxes; i++)
         if (axes[i] != xd) { yd = axes[i]; others++; }
-    if (others > 1) return -1; // the byte kernel convolves x and one more axis
+    if (others > 1) return -1; // the byte kernels convolve x and one more axis
     if (yd < 0) { // only x convolved: march down the untouched dimension of smallest stride
         for (int d = 0; d < nd; d++)
-            if (d != xd && src->extent[d] > 1 && (yd < 0 || std::llabs(src->stride[d]) < std::llabs(src->stride[yd])))
+            if (d != xd && !(rgb && d == 0) && src->extent[d] > 1 &&
+                (yd < 0 || std::llabs(src->stride[d]) < std::llabs(src->stride[yd])))
                 yd = d;
     }
-    const int64_t X = src->extent[xd];
-    if (X % 4 != 0 || X < 256 || X > (1 << 30)) return -1; // whole 256-pixel strips (the last one may overlap)
+    const int64_t X = src->extent[xd];                    // pixels (bytes unless rgb)
+    const int64_t row_bytes = rgb ? 3 * X : X;
+    if (X % 4 != 0 || X < 256 || X > (1 << 28)) return -1; // whole 256-pixel strips (the last one may overlap)
     const int64_t Y = yd >= 0 ? src->extent[yd] : 1;
     const int64_t sy = yd >= 0 ? src->stride[yd] : 0, dy = yd >= 0 ? dst->stride[yd] : 0;
     if (sy < 0 || dy < 0 || sy % 4 || dy % 4 || ((uintptr_t)src->data & 3) || ((uintptr_t)dst->data & 3)) return -1;
-    if ((Y - 1) * std::max(sy, dy) + X >= 0x7fffffffLL) return -1;
+    if ((Y - 1) * std::max(sy, dy) + row_bytes >= 0x7fffffffLL) return -1;
     const int kx = conv[xd] >= 0 ? k[conv[xd]].count : 1;
     const int ky = (yd >= 0 && conv[yd] >= 0) ? k[conv[yd]].count : 1;
-    if (!bytes_2d_supported(kx, ky)) return -1;
+    if (rgb ? !bytes_rgb_supported(kx, ky) : !bytes_2d_supported(kx, ky)) return -1;
     ByteArgs a;
     memset(&a, 0, sizeof a);
     a.src = (const uint8_t*)src->data;
@@ -856,13 +887,17 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
         const ndl_axis_kernel& ak = k[conv[xd]];
         a.cx = ak.center;
         a.bx = ak.border;
-        for (int t = 0; t < ak.count; t++) a.wx[t / 4] |= (uint32_t)ak.taps[t] << (8 * (t % 4));
+        for (int t = 0; t < ak.count; t++) {
+            a.wx[t / 4] |= (uint32_t)ak.taps[t] << (8 * (t % 4));
+            a.wxs[t] = (uint32_t)ak.taps[t];
+        }
     } else {
         a.cx = 0;
         a.bx = NDL_CLAMP;
         a.wx[0] = 1u;
+        a.wxs[0] = 1u;
     }
-    if (a.cx != kx / 2) return -1; // the kernel's x windows assume the default centre
+    if (a.cx != kx / 2) return -1; // the kernels' x windows assume the default centre
     if (yd >= 0 && conv[yd] >= 0) {
         const ndl_axis_kernel& ak = k[conv[yd]];
         a.cy = ak.center;
@@ -875,7 +910,7 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
     }
     int64_t batch = 1;
     for (int d = 0; d < nd; d++) {
-        if (d == xd || d == yd || src->extent[d] == 1) continue;
+        if (d == xd || d == yd || (rgb && d == 0) || src->extent[d] == 1) continue;
         if (src->stride[d] % 4 || dst->stride[d] % 4) return -1;
         a.bext[a.nb].init((uint32_t)src->extent[d]);
         a.sb[a.nb] = src->stride[d];
@@ -893,7 +928,7 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
     int64_t rows = std::max<int64_t>(8, (Y * cols + want - 1) / want);
     if (env_int("NDL_BYTES_ROWS", 0) > 0) rows = env_int("NDL_BYTES_ROWS", 0);
     // whole unrolled blocks: rows + ky - 1 input rows a multiple of the block
-    const int nu = bytes_prefetch_rows(ky);
+    const int nu = rgb ? bytes_rgb_prefetch_rows(ky) : bytes_prefetch_rows(ky);
     rows = (rows + ky - 1 + nu - 1) / nu * nu - (ky - 1);
     rows = std::min<int64_t>(rows, Y);
     a.rows = (int32_t)rows;
@@ -903,7 +938,7 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
     a.waves = (uint32_t)waves;
     a.fstrips.init((uint32_t)a.strips);
     a.fchunks.init((uint32_t)a.chunks);
-    hipError_t e = launch_bytes_2d(a, kx, ky, s);
+    hipError_t e = rgb ? launch_bytes_rgb(a, kx, ky, s) : launch_bytes_2d(a, kx, ky, s);
     if (e == hipErrorInvalidValue) return -1;
     if (e != hipSuccess) return hip_fail(e, "byte kernel launch");
     return NDL_OK;

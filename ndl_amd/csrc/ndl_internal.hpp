@@ -323,9 +323,16 @@ struct ByteArgs {
     FastDiv bext[NDL_MAX_DIM];
     int64_t sb[NDL_MAX_DIM], db[NDL_MAX_DIM];
     uint32_t wx[3];          // x taps, four 8-bit taps per word, window order, zero-padded
+    uint32_t wxs[9];         // x taps one per word (interleaved colour)
     uint32_t wy[9];          // y taps
 };
 bool bytes_2d_supported(int kx, int ky);
+// interleaved colour ({3, w, h}: channel fastest, x stride 3 bytes): X is
+// the pixel count of a row (a multiple of 4, at least 256), x convolved
+// with kx in {3, 5, 7, 9} taps, ky in {1, 3, 5, 7, 9}
+bool bytes_rgb_supported(int kx, int ky);
+constexpr int bytes_rgb_prefetch_rows(int ky) { return ky >= 5 ? ky : (ky == 3 ? 3 : 4); }
+hipError_t launch_bytes_rgb(const ByteArgs& a, int kx, int ky, hipStream_t s);
 // input rows a wave keeps in flight for ky y taps; a chunk whose rows + ky - 1
 // input rows are a multiple of it runs with no per-row checks
 constexpr int bytes_prefetch_rows(int ky) { return ky >= 5 ? 2 * ky : (ky == 3 ? 12 : 16); }

@@ -758,6 +758,55 @@ def test_u8_byte_kernel_bit_exact(case):
     np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, REF), want)
 
 
+# interleaved colour ({3, w, h}: ndl axis 0 the channel, 1 the pixel with
+# stride 3, 2 the row): x convolved -> the colour form of the byte kernel;
+# only other axes -> channel and pixel merge into one byte axis
+RGB_CASES = [
+    # (numpy shape, ndl axes, x taps, other taps, x border, other border)
+    ((37, 260, 3), [1, 2], np.ones(3), np.ones(3), "clamp", "reflect"),  # last strip overlaps the first
+    ((64, 1024, 3), [1, 2], np.array([1, 4, 6, 4, 1.0]), np.array([1, 4, 6, 4, 1.0]), "reflect", "wrap"),
+    ((23, 516, 3), [1, 2], np.array([255, 0, 17, 3, 200, 9, 1.0]), np.array([7, 255, 1, 0, 99, 2, 5.0]),
+     "wrap", "clamp"),
+    ((19, 256, 3), [1, 2], np.arange(1, 10.0), np.arange(9, 0, -1.0), "reflect", "reflect"),  # one strip
+    ((4, 256, 3), [1, 2], np.ones(9), np.ones(9), "clamp", "wrap"),   # rows fewer than taps
+    ((33, 400, 3), [1], np.array([3, 1, 2, 5, 4, 1, 9.0]), None, "wrap", None),  # x only
+    ((33, 400, 3), [2], None, np.arange(1, 8.0), None, "reflect"),     # rows only: one byte axis
+    ((4, 30, 300, 3), [1, 2], np.ones(5), np.ones(5), "reflect", "clamp"),  # frames as a batch axis
+    ((6, 20, 288, 3), [1, 3], np.ones(7), np.ones(3), "wrap", "reflect"),   # x and frames
+]
+
+
+@pytest.mark.parametrize("case", range(len(RGB_CASES)))
+def test_u8_rgb_byte_kernel_bit_exact(case):
+    shape, axes, tx, ty, bx, by = RGB_CASES[case]
+    rng = np.random.default_rng(80 + case)
+    x = rng.integers(0, 256, size=shape, dtype=np.uint8)
+    taps, borders = [], []
+    for a in axes:
+        t, b = (tx, bx) if a == 1 else (ty, by)
+        taps.append(t)
+        borders.append(b)
+    say("u8 rgb byte kernel", case, shape, axes)
+    want = oracle.sepconv(x, axes, taps, borders)
+    np.testing.assert_array_equal(gpu_sep(x, axes, taps, borders, REF), want)
+
+
+def test_u8_rgb_views_and_fallbacks():
+    rng = np.random.default_rng(90)
+    x = rng.integers(0, 256, size=(3, 40, 300, 3), dtype=np.uint8)
+    box = [np.ones(5), np.ones(5)]
+    # frames reversed (negative batch stride)
+    say("u8 rgb view")
+    want = oracle.sepconv(np.ascontiguousarray(x[::-1]), [1, 2], box, ["wrap", "reflect"])
+    np.testing.assert_array_equal(gpu_sep(x, [1, 2], box, ["wrap", "reflect"], REF, view=np.s_[::-1]), want)
+    # narrower than a strip, pixel count not a multiple of 4: the float paths
+    for w in (132, 301):
+        xx = x[0, :, :w].copy()
+        say("u8 rgb fallback", xx.shape)
+        want = oracle.sepconv(xx, [1, 2], box, ["clamp", "wrap"])
+        np.testing.assert_array_equal(gpu_sep(xx, [1, 2], box, ["clamp", "wrap"], REF), want)
+
+
 def test_u8_byte_kernel_views_and_fallbacks():
     rng = np.random.default_rng(60)
     x = rng.integers(0, 256, size=(3, 50, 300), dtype=np.uint8)
