@@ -413,11 +413,29 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
             bitems.append(it)
             keep.append(ks)
         arr = (abi.BatchItem * len(bitems))(*bitems)
-        plan = C.c_void_p()
-        ndl_amd.check(lib.ndl_plan_create(arr, len(bitems), abi.ACCUM_F32, C.byref(plan)))
-        run_plan = lambda: ndl_amd.check(lib.ndl_plan_execute(plan, stream))  # noqa: E731
-        run_plan()
-        ms_plan = _time_launches(hip, stream, run_plan, max(3, reps // 2), spin)
+
+        def time_plan(env=None):
+            """one plan (autotuned, or pinned by an environment knob), ms per execute"""
+            saved = {k: os.environ.get(k) for k in (env or {})}
+            os.environ.update(env or {})
+            try:
+                plan = C.c_void_p()
+                ndl_amd.check(lib.ndl_plan_create(arr, len(bitems), abi.ACCUM_F32, C.byref(plan)))
+            finally:
+                for k, v in saved.items():
+                    if v is None:
+                        os.environ.pop(k, None)
+                    else:
+                        os.environ[k] = v
+            run_plan = lambda: ndl_amd.check(lib.ndl_plan_execute(plan, stream))  # noqa: E731
+            run_plan()  # autotunes (and captures) on first execute
+            ms = _time_launches(hip, stream, run_plan, max(3, reps // 2), spin)
+            ndl_amd.check(lib.ndl_plan_destroy(plan))
+            return ms
+
+        ms_queues = time_plan({"NDL_PLAN_PER_AXIS": "0"})
+        ms_axes = time_plan({"NDL_PLAN_PER_AXIS_ONLY": "1"})
+        ms_plan = time_plan()
         singles = [fwd_fn(srcs[i], dsts[i], list(range(len(items[i][0]))), items[i][1],
                           [items[i][2]] * len(items[i][0]), abi.ACCUM_F32) for i in range(len(items))]
 
@@ -426,9 +444,9 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
                 f()
         run_singles()
         ms_single = _time_launches(hip, stream, run_singles, max(3, reps // 2), spin)
-        ndl_amd.check(lib.ndl_plan_destroy(plan))
-        emit({"config": "4: 256 uneven f32 instances (192 2-D, 64 3-D), one work-queue launch (hipGraph)",
+        emit({"config": "4: 256 uneven f32 instances (192 2-D, 64 3-D), one plan (autotuned, hipGraph)",
               "voxels": nvox, "plan_us": round(ms_plan * 1e3, 1), "plan_gvox_s": round(nvox / ms_plan / 1e6, 2),
+              "work_queues_only_us": round(ms_queues * 1e3, 1), "per_axis_only_us": round(ms_axes * 1e3, 1),
               "single_launches_us": round(ms_single * 1e3, 1),
               "speedup_vs_single_launches": round(ms_single / ms_plan, 3),
               "GBps_alg_plan": round(8 * nvox / ms_plan / 1e6, 1)})

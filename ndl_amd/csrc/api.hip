@@ -1884,11 +1884,13 @@ NDL_API int ndl_version(void) { return 100; }
 // A plan groups its fused-eligible items by storage type and z tap count
 // (the register ring's compile-time period) into device job tables;
 // execute() runs one persistent work-queue launch per group, then any
-// remaining items through the per-axis chain.  The z-chunk length of the
-// jobs is autotuned: every candidate is built at creation, the first
-// execute times each with HIP events on the caller's stream and keeps the
-// fastest, whose launches are then captured once into a hipGraph and
-// replayed.
+// remaining items through the per-axis chain.  Candidate schedules: work
+// queues with z chunks of 32, 64, 128 or whole, one fused launch per item,
+// and (FP32 items) each item as a chain of one-axis passes through a
+// plan-owned buffer, the items dealt over four side streams.  Every
+// candidate is built at creation; the first execute captures each into a
+// hipGraph, times its replays with HIP events on the caller's stream and
+// keeps the fastest graph, which later executes replay.
 namespace {
 struct PlanGroup {
     int dt = 0;
@@ -1913,6 +1915,9 @@ struct PlanVariant {
     std::vector<PlanGroup> groups;
     bool singles = false;       // one fused launch per item instead of work queues
     std::vector<PlanSingle> one;
+    bool per_axis = false;      // each item as a chain of one-axis passes
+    std::vector<std::vector<int>> lanes; // per side stream: its items, largest first
+    std::vector<void*> temp;    // per side stream: FP32 buffer of its largest item
     float ms = -1.f;            // measured time of one execute
     void release()
     {
@@ -1920,6 +1925,10 @@ struct PlanVariant {
             if (g.d_jobs) (void)hipFree(g.d_jobs);
             if (g.d_ts) (void)hipFree(g.d_ts);
             g.d_jobs = g.d_ts = nullptr;
+        }
+        for (void*& t : temp) {
+            if (t) (void)hipFree(t);
+            t = nullptr;
         }
     }
 };
@@ -1940,9 +1949,8 @@ struct ndl_plan {
     std::vector<hipEvent_t> join;
     hipEvent_t fork = nullptr;
     hipStream_t cap = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;     // the chosen candidate's capture
     hipGraphExec_t exec = nullptr;
-    bool tried_graph = false;
     ~ndl_plan()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1957,8 +1965,42 @@ struct ndl_plan {
 };
 
 namespace {
+// one item as one-axis passes: src -> T -> dst (two axes) or src -> dst ->
+// T -> dst (three), T a dense FP32 buffer; each pass takes the one-axis
+// kernels where they apply (forward_impl with one axis)
+int run_item_per_axis(const ndl_plan* p, const ndl_batch_item& it, void* temp, hipStream_t q)
+{
+    const ndl_tensor T = dense_like(&it.src, temp, NDL_F32);
+    const ndl_tensor* in = &it.src;
+    for (int a = 0; a < it.naxes; a++) {
+        const int left = it.naxes - 1 - a; // passes after this one
+        const ndl_tensor* out = (left % 2 == 0) ? &it.dst : &T;
+        int rc = forward_impl(in, out, 1, &it.axes[a], &it.k[a], p->accum, NDL_ROUND_TRUNC, q);
+        if (rc) return rc;
+        in = out;
+    }
+    return NDL_OK;
+}
+
 int launch_groups(ndl_plan* p, const PlanVariant& v, hipStream_t s)
 {
+    if (v.per_axis) {
+        // lanes run side by side, each waiting for the caller's prior work
+        // and the caller waiting for all of them
+        const size_t nl = v.lanes.size();
+        if (hipEventRecord(p->fork, s) != hipSuccess) return hip_fail(hipGetLastError(), "fork event");
+        for (size_t j = 0; j < nl; j++) {
+            hipStream_t q = p->side[j];
+            if (hipStreamWaitEvent(q, p->fork, 0) != hipSuccess) return hip_fail(hipGetLastError(), "fork wait");
+            for (int i : v.lanes[j]) {
+                int rc = run_item_per_axis(p, p->items[i], v.temp[j], q);
+                if (rc) return rc;
+            }
+            if (hipEventRecord(p->join[j], q) != hipSuccess || hipStreamWaitEvent(s, p->join[j], 0) != hipSuccess)
+                return hip_fail(hipGetLastError(), "join");
+        }
+        return NDL_OK;
+    }
     for (const PlanSingle& o : v.one) {
         hipError_t e = launch_fused(o.fa, o.cfg, o.dt, o.int_acc, s);
         if (e != hipSuccess) return hip_fail(e, "fused launch");
@@ -2067,35 +2109,129 @@ int build_singles(ndl_plan* p, PlanVariant& v)
     return NDL_OK;
 }
 
-// first execute: time every candidate (one warm-up, three timed runs) and
-// keep the fastest
+// the per-axis candidate (FP32 accumulation, float32 items): every fusable
+// item as one-axis passes, the items dealt largest first to the least
+// loaded of `nlanes` side streams, one FP32 buffer per stream.  Returns
+// false (and builds nothing) when it does not apply.
+bool build_per_axis(ndl_plan* p, PlanVariant& v, int nlanes)
+{
+    if (p->accum != NDL_ACCUM_F32 || nlanes < 1) return false;
+    std::vector<int> order;
+    for (int i = 0; i < (int)p->items.size(); i++) {
+        const ndl_batch_item& it = p->items[i];
+        if (numel(&it.src) == 0 || std::find(p->fallback.begin(), p->fallback.end(), i) != p->fallback.end()) continue;
+        if (it.src.dtype != NDL_F32 || it.dst.dtype != NDL_F32) return false;
+        order.push_back(i);
+    }
+    if (order.empty()) return false;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return numel(&p->items[a].src) > numel(&p->items[b].src); });
+    v.per_axis = true;
+    v.lanes.assign(nlanes, {});
+    std::vector<int64_t> load(nlanes, 0), biggest(nlanes, 0);
+    for (int i : order) {
+        const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        const int64_t n = numel(&p->items[i].src);
+        v.lanes[j].push_back(i);
+        load[j] += n * p->items[i].naxes;
+        biggest[j] = std::max(biggest[j], n);
+    }
+    v.temp.assign(nlanes, nullptr);
+    for (int j = 0; j < nlanes; j++) {
+        if (biggest[j] == 0) continue;
+        if (hipMalloc(&v.temp[j], (size_t)biggest[j] * sizeof(float)) != hipSuccess) {
+            (void)hipGetLastError();
+            v.release();
+            v = PlanVariant();
+            return false;
+        }
+    }
+    return true;
+}
+
+// capture one candidate's launches into an executable graph on the plan's
+// private stream; nullptr when capture or instantiation fails (the
+// candidate then runs as plain launches)
+hipGraphExec_t capture_variant(ndl_plan* p, const PlanVariant& v, hipGraph_t* gout)
+{
+    *gout = nullptr;
+    if (!env_int("NDL_PLAN_GRAPH", 1)) return nullptr;
+    if (!p->cap && hipStreamCreate(&p->cap) != hipSuccess) {
+        p->cap = nullptr;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const int rc = launch_groups(p, v, p->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(p->cap, &g);
+    hipGraphExec_t x = nullptr;
+    if (rc == NDL_OK && e == hipSuccess && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
+        *gout = g;
+        return x;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return nullptr;
+}
+
+int run_variant(ndl_plan* p, const PlanVariant& v, hipGraphExec_t x, hipStream_t s)
+{
+    if (x) {
+        const hipError_t e = hipGraphLaunch(x, s);
+        return e == hipSuccess ? NDL_OK : hip_fail(e, "hipGraphLaunch");
+    }
+    return launch_groups(p, v, s);
+}
+
+// first execute: capture every candidate, time its replays (one warm-up,
+// three timed runs, waiting for the timings: the one call that blocks the
+// host) and keep the fastest graph
 int autotune(ndl_plan* p, hipStream_t s)
 {
-    if (p->variants.size() == 1) {
-        p->chosen = 0;
-        return NDL_OK;
-    }
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        p->chosen = 0;
-        return NDL_OK;
-    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = p->variants.size() > 1 && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
     int best = 0;
-    for (int v = 0; v < (int)p->variants.size(); v++) {
-        int rc = launch_groups(p, p->variants[v], s);
-        if (rc) return rc;
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < 3; r++)
-            if ((rc = launch_groups(p, p->variants[v], s))) return rc;
-        (void)hipEventRecord(e1, s);
-        float ms = 0;
-        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
-        p->variants[v].ms = ms / 3;
-        if (p->variants[v].ms < p->variants[best].ms) best = v;
+    hipGraph_t best_g = nullptr;
+    hipGraphExec_t best_x = nullptr;
+    int rc = NDL_OK;
+    for (int v = 0; v < (int)p->variants.size() && rc == NDL_OK; v++) {
+        PlanVariant& pv = p->variants[v];
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = capture_variant(p, pv, &g);
+        if (timed) {
+            rc = run_variant(p, pv, x, s);
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3 && rc == NDL_OK; r++) rc = run_variant(p, pv, x, s);
+            (void)hipEventRecord(e1, s);
+            float ms = 0;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
+            pv.ms = ms / 3;
+        }
+        if (v == 0 || (timed && pv.ms < p->variants[best].ms)) {
+            if (best_x) (void)hipGraphExecDestroy(best_x);
+            if (best_g) (void)hipGraphDestroy(best_g);
+            best = v;
+            best_g = g;
+            best_x = x;
+        } else {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+        }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) {
+        if (best_x) (void)hipGraphExecDestroy(best_x);
+        if (best_g) (void)hipGraphDestroy(best_g);
+        return rc;
+    }
     p->chosen = best;
+    p->graph = best_g;
+    p->exec = best_x;
     for (int v = 0; v < (int)p->variants.size(); v++)
         if (v != best) p->variants[v].release();
     return NDL_OK;
@@ -2133,17 +2269,31 @@ NDL_API int ndl_plan_create(const ndl_batch_item* items, int32_t n, int32_t accu
         if (rc) return rc;
         max_groups = std::max(max_groups, p->variants.back().groups.size());
     }
+    const int nstreams = std::max(1, env_int("NDL_PLAN_STREAMS", 4));
+    bool per_axis = false;
     if (forced <= 0) {
         p->variants.emplace_back();
         int rc = build_singles(p.get(), p->variants.back());
         if (rc) return rc;
+        if (env_int("NDL_PLAN_PER_AXIS", 1)) {
+            p->variants.emplace_back();
+            per_axis = build_per_axis(p.get(), p->variants.back(), nstreams);
+            if (!per_axis) {
+                p->variants.pop_back();
+            } else if (env_int("NDL_PLAN_PER_AXIS_ONLY", 0)) {
+                // pin the per-axis candidate (tests and measurement)
+                for (size_t i = 0; i + 1 < p->variants.size(); i++) p->variants[i].release();
+                p->variants.erase(p->variants.begin(), p->variants.end() - 1);
+                max_groups = 1;
+            }
+        }
     }
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     p->grid = (uint32_t)std::max(1, cus) * (uint32_t)env_int("NDL_PLAN_WG_PER_CU", 4);
     hipError_t e = hipMalloc((void**)&p->d_counter, sizeof(uint32_t) * max_groups);
     if (e != hipSuccess) return fail(NDL_ERR_NOMEM, "plan counter: %s", hipGetErrorString(e));
-    const int nside = max_groups > 1 ? std::min<int>((int)max_groups, env_int("NDL_PLAN_STREAMS", 4)) : 0;
+    const int nside = per_axis ? nstreams : (max_groups > 1 ? std::min<int>((int)max_groups, nstreams) : 0);
     if (nside > 0) {
         if ((e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming)) != hipSuccess)
             return hip_fail(e, "plan fork event");
@@ -2169,32 +2319,9 @@ NDL_API int ndl_plan_execute(ndl_plan* p, void* stream)
         if (rc) return rc;
     }
     const PlanVariant& v = p->variants[p->chosen];
-    if (!v.groups.empty() || !v.one.empty()) {
-        if (!p->tried_graph && env_int("NDL_PLAN_GRAPH", 1)) {
-            // capture the work-queue launches once on a private stream
-            p->tried_graph = true;
-            if (hipStreamCreate(&p->cap) == hipSuccess &&
-                hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                int rc = launch_groups(p, v, p->cap);
-                hipGraph_t g = nullptr;
-                hipError_t e = hipStreamEndCapture(p->cap, &g);
-                if (rc == NDL_OK && e == hipSuccess && g &&
-                    hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) == hipSuccess) {
-                    p->graph = g;
-                } else {
-                    if (g) (void)hipGraphDestroy(g);
-                    p->exec = nullptr;
-                }
-            }
-            (void)hipGetLastError();
-        }
-        if (p->exec) {
-            hipError_t e = hipGraphLaunch(p->exec, s);
-            if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
-        } else {
-            int rc = launch_groups(p, v, s);
-            if (rc) return rc;
-        }
+    if (!v.groups.empty() || !v.one.empty() || v.per_axis) {
+        int rc = run_variant(p, v, p->exec, s);
+        if (rc) return rc;
     }
     for (int i : p->fallback) {
         const ndl_batch_item& it = p->items[i];

@@ -10,6 +10,7 @@ so the results are bit-identical to the serial oracle.
     wrap), FP32 within 1e-6 of sum|w||x| and reference arithmetic bit-exact;
   * config 4: all 256 uneven instances in one plan, once with the z chunk
     forced (work-queue variant, every group of the full plan runs), once
+    pinned to the per-axis variant (one-axis passes on four streams), once
     autotuned, each instance against the oracle, over three graph replays;
   * config 5: tap gradients with bf16 / f16 storage of x at 384^3 against
     the FP64 oracle and FP64 central differences (step 1e-3) of the
@@ -100,7 +101,7 @@ def _config4_plan(items, values):
     return plan, srcs, dsts, (arr, keep)
 
 
-@pytest.mark.parametrize("variant", ["work_queue_tz32", "autotuned"])
+@pytest.mark.parametrize("variant", ["work_queue_tz32", "per_axis", "autotuned"])
 def test_config4_full_plan(variant, monkeypatch):
     items = workloads.config4()
     say("config4", variant, ": inputs and oracle for 256 instances")
@@ -114,6 +115,9 @@ def test_config4_full_plan(variant, monkeypatch):
         # one candidate, the persistent work queues: every (tile shape, ring
         # period) group of the full plan runs, and nothing overwrites it
         monkeypatch.setenv("NDL_PLAN_TZ", "32")
+    elif variant == "per_axis":
+        # one candidate, every instance as one-axis passes on four streams
+        monkeypatch.setenv("NDL_PLAN_PER_AXIS_ONLY", "1")
     plan, srcs, dsts, keep = _config4_plan(items, values)
     lib = ndl_amd.lib()
     try:
