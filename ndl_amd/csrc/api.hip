@@ -944,6 +944,69 @@ int try_bytes(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const
     return NDL_OK;
 }
 
+// The whole-row kernel (kernels_slab.hip) for float32 volumes whose rows
+// fit one tile: a workgroup owns every column of 4 rows and a long z chunk,
+// so only the z halo of the chunk is filtered twice.  Returns -1 when it
+// does not apply (the tile kernel or the chain runs instead).
+int try_slab(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
+             const ndl_axis_kernel* k, int32_t accum, hipStream_t s)
+{
+    if (!env_int("NDL_SLAB", 1) || accum != NDL_ACCUM_F32 || src->dtype != NDL_F32 || dst->dtype != NDL_F32)
+        return -1;
+    FusedArgs fa;
+    FusedConfig cfg;
+    bool int_acc = false;
+    int rd[3];
+    if (!plan_fused(src, dst, naxes, axes, k, accum, fa, cfg, int_acc, 0, false, rd)) return -1;
+    if (cfg.ty == 1 || cfg.il != 1 || rd[1] < 0 || rd[2] < 0) return -1;
+    if (!slab_supported(fa.kx, fa.ky, fa.kz) || fa.cx != (fa.kx - 1) / 2) return -1;
+    // rows of one tile: x extent even (8-byte output pairs), at most 256,
+    // and at least kx - 1 (one halo fold per lane)
+    if (fa.X > 256 || fa.X % 2 || fa.X < fa.kx - 1) return -1;
+    // 32-bit byte offsets: rows at a non-negative stride, slices either way
+    if (fa.sy < 0 || (int64_t)(fa.Y - 1) * fa.sy * 4 + (int64_t)fa.X * 4 >= 0x7fffffffLL) return -1;
+    if (std::llabs(fa.sz) * (int64_t)(fa.Z - 1) * 4 >= 0x7fffffffLL) return -1;
+    // 8-byte aligned output pairs
+    if ((uintptr_t)dst->data % 8 || (fa.dy & 1) || (fa.dz & 1)) return -1;
+    for (int d = 0; d < fa.nb; d++)
+        if (fa.db[d] & 1) return -1;
+    int64_t batch = 1;
+    for (int d = 0; d < fa.nb; d++) batch *= fa.bext[d];
+    fa.tiles_x = 1;
+    fa.tiles_y = (fa.Y + 3) / 4;
+    const int64_t rows = (int64_t)fa.tiles_y * batch;
+    // z chunks: tiles fill the resident workgroup slots in whole rounds; a
+    // chunk of tz outputs costs tz + kz - 1 slice steps
+    static const int per_cu = std::max(1, slab_blocks_per_cu(11, 11, 19));
+    const int64_t slots = (int64_t)device_cus() * per_cu;
+    int tz = fa.Z;
+    double best = 1e300;
+    for (int chunks = 1; chunks <= std::min(fa.Z, 256); chunks++) {
+        const int t = (fa.Z + chunks - 1) / chunks;
+        const int64_t blocks = rows * ((fa.Z + t - 1) / t);
+        const double cost = (double)((blocks + slots - 1) / slots) * (t + fa.kz - 1);
+        if (cost < best - 1e-9) {
+            best = cost;
+            tz = t;
+        }
+    }
+    const int tz_env = env_int("NDL_SLAB_TZ", 0);
+    if (tz_env > 0) tz = std::min(tz_env, fa.Z);
+    fa.tz = tz;
+    fa.tiles_z = (fa.Z + tz - 1) / tz;
+    const int64_t ntiles = rows * fa.tiles_z;
+    if (ntiles <= 0 || ntiles > 0x7fffffffLL) return -1;
+    fa.ntiles = (uint32_t)ntiles;
+    fa.fdx.init(1u);
+    fa.fdy.init((uint32_t)fa.tiles_y);
+    fa.fdz.init((uint32_t)fa.tiles_z);
+    fa.xcd_remap = (ntiles % 8 == 0) && env_int("NDL_XCD_REMAP", 1);
+    hipError_t e = launch_slab(fa, s);
+    if (e == hipErrorInvalidValue) return -1;
+    if (e != hipSuccess) return hip_fail(e, "slab launch");
+    return NDL_OK;
+}
+
 int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                  const ndl_axis_kernel* k, int32_t accum, int32_t rounding, hipStream_t s)
 {
@@ -953,6 +1016,10 @@ int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, co
     }
     if (naxes == 1) {
         const int rc = try_one_axis(src, dst, axes[0], k[0], accum, s);
+        if (rc != -1) return rc;
+    }
+    if (naxes == 3) {
+        const int rc = try_slab(src, dst, naxes, axes, k, accum, s);
         if (rc != -1) return rc;
     }
     FusedArgs fa;

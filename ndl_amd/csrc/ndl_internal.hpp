@@ -216,6 +216,14 @@ struct FusedQueue {
 hipError_t launch_fused_queue(const FusedQueue& q, const FusedConfig& cfg, int dt, bool int_acc, size_t lds,
                               hipStream_t s);
 int fused_ring_for(int kz);       // ring period for kz, or -1 when unsupported
+// Whole-row fused kernel (kernels_slab.hip): float32 storage and FP32 sums,
+// x (unit stride) extent even and at most 256, three axes with the
+// compile-time tap counts of an instance, x kernel centred.  FusedArgs as
+// for the tile kernel with tiles_x = 1, tiles_y = ceil(Y / 4) and tz the z
+// chunk; one workgroup of 768 threads per tile.
+bool slab_supported(int kx, int ky, int kz);
+int slab_blocks_per_cu(int kx, int ky, int kz);
+hipError_t launch_slab(const FusedArgs& a, hipStream_t s);
 // Instances with edge weight rows (cfg.edge), kernels_fused_edge.hip.
 hipError_t launch_fused_edge(const FusedArgs& a, const FusedConfig& cfg, int dt, size_t lds, hipStream_t s);
 bool fused_edge_supported(const FusedConfig& cfg, int dt, int kx, int ky);
