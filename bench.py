@@ -65,7 +65,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+              "(sharding over several GPUs is designed, not built or measured)", file=sys.stderr)
         sys.exit(2)
 
     import ndl_amd
@@ -193,17 +193,18 @@ def main():
             "cache": "cold (1 GiB read flush before each step)" if cold else "warm",
         },
         "roofline": {
-            # the roofline denominator is HBM; counters show the kernel itself is
-            # bound by VALU issue and LDS (see DESIGN.md's measurement notes)
+            # the roofline denominator is HBM; the SQ counters of the kernel
+            # (profiles/) show it bound by instruction issue and the per-slice
+            # barrier's latency, not by HBM bandwidth
             "bound": "hbm",
-            "measured_limiter": "valu+lds",
+            "measured_limiter": "valu issue + per-slice barrier latency",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "fused_tile_kernel (fwd and adjoint launches)",
+            "kernel": "slab_kernel (fwd and adjoint launches)",
             "alg_bytes_per_launch": alg_bytes,
             "mean_launch_us": round(launch_ms * 1e3, 2),
         },
