@@ -194,10 +194,10 @@ def main():
         },
         "roofline": {
             # the roofline denominator is HBM; the SQ counters of the kernel
-            # (profiles/) show it bound by instruction issue and the per-slice
-            # barrier's latency, not by HBM bandwidth
+            # (profiles/) show it bound by instruction issue and memory
+            # latency, not by HBM bandwidth (warm and cold times are equal)
             "bound": "hbm",
-            "measured_limiter": "valu issue + per-slice barrier latency",
+            "measured_limiter": "instruction issue and memory latency (SQ counters), not HBM bandwidth",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",

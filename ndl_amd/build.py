@@ -7,6 +7,7 @@ hipcc cross-compiles without a GPU.  Every HIP source is compiled with
 fmaf explicitly) and with packed-FP32 instructions disabled.
 """
 import os
+import re
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -27,12 +28,24 @@ FLAGS = [
 ]
 
 
+def _includes(path, seen):
+    """path and every file it includes with quotes, recursively"""
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    for line in open(path):
+        m = re.match(r'\s*#\s*include\s+"([^"]+)"', line)
+        if m:
+            _includes(os.path.join(os.path.dirname(path), m.group(1)), seen)
+    return seen
+
+
 def _stale(src, out):
-    """NDL_BUILD_INCREMENTAL=1 skips objects newer than every source and header."""
+    """NDL_BUILD_INCREMENTAL=1 skips objects newer than their source and every header it includes."""
     if os.environ.get("NDL_BUILD_INCREMENTAL") != "1" or not os.path.exists(out):
         return True
-    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
-    deps.append(os.path.join(HERE, "..", "include", "ndl_gpu.h"))
+    deps = _includes(os.path.join(CSRC, src), set())
     return max(os.path.getmtime(d) for d in deps) > os.path.getmtime(out)
 
 

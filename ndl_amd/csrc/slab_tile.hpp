@@ -3,8 +3,7 @@
 //
 // A workgroup owns every column of TY = 4 consecutive rows (a row of X <=
 // 256 elements) and a chunk of tz output slices along z.  Its waves take one
-// of two roles and run a two-stage pipeline, one barrier per slice step
-// (step s: y-pass of slice s, x- and z-pass of slice s-1):
+// of two roles:
 //   Y waves: one column and the 4 rows per lane.  The 4 + ky - 1 input rows
 //            of a slice are read straight from global memory (buffer loads,
 //            consecutive lanes on consecutive columns), D slices ahead into
@@ -15,8 +14,13 @@
 //   XZ waves: two consecutive columns of one row per lane: the x-pass from
 //            8-byte LDS reads of the y-plane window, then a register ring of
 //            the last kz x-filtered slices; once kz slices are in, one z
-//            output pair per step, one 8-byte store (512 contiguous bytes per
-//            wave).
+//            output pair per slice, one 8-byte store (512 contiguous bytes
+//            per wave).
+// The y-planes form a ring of NR buffers in LDS.  The roles synchronise per
+// buffer with LDS counters instead of a workgroup barrier: Y waves count the
+// writes of each buffer, XZ waves its reads, and each side waits (bounded
+// spin) only for the buffer it needs, so a Y wave whose loads arrive late
+// stalls the XZ waves only once they have used up the NR - 1 planes ahead.
 // Only the z halo of a chunk (kz - 1 slices) is filtered twice; with one
 // long chunk per workgroup that is the only recomputation of the pass.
 //
@@ -32,18 +36,40 @@ using fused_detail::static_for;
 constexpr int kTY = 4;     // rows per tile
 constexpr int kTXW = 256;  // columns per tile row
 
-// s_waitcnt immediates (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
-// expcnt 6:4, lgkmcnt 11:8)
-constexpr int waitcnt_vm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0f70; }
+// s_waitcnt immediate for lgkmcnt(0) (gfx9 encoding: vmcnt in bits 3:0 and
+// 15:14, expcnt 6:4, lgkmcnt 11:8)
 constexpr int kWaitLgkm0 = 0xc07f;
 
-// s_barrier after this wave's LDS operations completed; compiler-level
-// memory barriers on both sides keep every LDS access on its side
-__device__ __forceinline__ void step_barrier()
+// Wait until the LDS counter at cnt reaches target (other waves of the
+// workgroup add to it).  The spin is bounded (2^21 polls with a short sleep,
+// far beyond any real wait) so no wave can spin forever, and written in asm
+// so the register allocator sees straight-line code around it.
+__device__ __forceinline__ void wait_ge(const int* cnt, int target)
+{
+    const uint32_t addr = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const int*)cnt;
+    int v, sv, it;
+    __asm__ volatile(
+        "s_mov_b32 %[it], 0x200000\n"
+        "1:\n\t"
+        "ds_read_b32 %[v], %[addr]\n\t"
+        "s_waitcnt lgkmcnt(0)\n\t"
+        "v_readfirstlane_b32 %[s], %[v]\n\t"
+        "s_cmp_ge_i32 %[s], %[tgt]\n\t"
+        "s_cbranch_scc1 2f\n\t"
+        "s_sleep 1\n\t"
+        "s_sub_u32 %[it], %[it], 1\n\t"
+        "s_cmp_eq_u32 %[it], 0\n\t"
+        "s_cbranch_scc0 1b\n"
+        "2:"
+        : [v] "=&v"(v), [s] "=&s"(sv), [it] "=&s"(it)
+        : [addr] "v"(addr), [tgt] "s"(target)
+        : "scc", "memory");
+}
+__device__ __forceinline__ void signal(int* cnt)
 {
     __asm__ volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0); // this wave's LDS writes are done
+    if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __asm__ volatile("" ::: "memory");
 }
 
@@ -83,10 +109,14 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
     constexpr int YP = W::PITCH;
     constexpr int NXW = KX + 1;           // x window of two outputs
     constexpr int PLANE = TY * YP;        // floats per y-plane
+    constexpr int NR = 4;                 // y-planes in the LDS ring
     static_assert(KX % 2 == 1 && KX <= 17, "odd x kernels up to 17 taps");
     static_assert((D - 1) * NRY <= 63, "loads in flight exceed the wait counter");
-    static_assert(D % 2 == 0, "even prefetch depth: compile-time y-plane parity");
-    __shared__ __align__(16) float ypl[2 * PLANE];
+    __shared__ __align__(16) float ypl[NR * PLANE];
+    // per ring buffer: Y waves that wrote it / XZ waves that read it, summed
+    // over its uses (use g of buffer b is slice g * NR + b)
+    __shared__ int ycnt[NR];
+    __shared__ int xcnt[NR];
 
     // tile -> (y tile, z chunk, batch), y fastest
     uint32_t q = a.fdy.div(tile);
@@ -104,15 +134,15 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
     const int y0 = tiy * TY, z0 = tiz * a.tz;
     const int nz = min(a.tz, a.Z - z0);
     const int nsl = nz + KZ - 1; // input slices of the chunk
-    // pipeline steps: nsl + 1 (y-pass of slice s, x- and z-pass of s - 1),
-    // rounded up to whole groups of D; every wave passes the same NSP barriers
-    const int NSP = (nsl + 1 + D - 1) / D * D;
     const int X = a.X;
     const float* src = (const float*)a.src + soff;
     float* dst = (float*)a.dst + doff;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
     const bool z_interior = z0 - a.cz >= 0 && z0 - a.cz + nsl <= a.Z;
+    if (threadIdx.x < NR) ycnt[threadIdx.x] = 0;
+    if (threadIdx.x < NR) xcnt[threadIdx.x] = 0;
+    __syncthreads();
 
     if (w < NWY) {
         // ------------------------------------------------------ Y role
@@ -159,7 +189,7 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
                 buf[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, coff, roff[i], 0));
         };
         // y-pass of one slice into y-plane `par` (compile-time parity)
-        auto ypass = [&](const float (&in)[NRY], auto par) NDL_INLINE {
+        auto ypass = [&](const float (&in)[NRY], int par) NDL_INLINE {
             float o[TY];
 #pragma unroll
             for (int r = 0; r < TY; r++) o[r] = 0.f;
@@ -169,7 +199,7 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
 #pragma unroll
                 for (int r = 0; r < TY; r++) o[r] = __builtin_fmaf(in[r + t], wt, o[r]);
             }
-            float* yp = ypl + decltype(par)::value * PLANE;
+            float* yp = ypl + par * PLANE;
 #pragma unroll
             for (int r = 0; r < TY; r++) yp[r * YP + own] = o[r];
 #pragma unroll
@@ -193,26 +223,21 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
         float pre[D][NRY];
 #pragma unroll
         for (int u = 0; u < D; u++) load(u, pre[u]);
-        int s0 = 0;
-        // steady state: every step of the block y-filters a slice (D is
-        // even, so the y-plane of step s0 + u is u & 1)
-        for (; s0 + D <= nsl; s0 += D) {
-            static_for<D>([&](auto ic) NDL_INLINE {
-                constexpr int u = decltype(ic)::value;
-                ypass(pre[u], IC<u & 1>{});
-                load(s0 + u + D, pre[u]);
-                step_barrier();
-            });
-        }
-        // tail (fewer than 2 D + 2 steps; the buffer of step s stays s % D)
-        static_for<2 * D + 2>([&](auto ic) NDL_INLINE {
+        // slice sl: once every XZ wave has read the previous use of its
+        // buffer, the y-pass into it; then the loads of slice sl + D
+        auto yslice = [&](auto ic, int sl) NDL_INLINE {
             constexpr int u = decltype(ic)::value;
-            const int s = s0 + u;
-            if (s < NSP) {
-                if (s < nsl) ypass(pre[u % D], IC<u & 1>{});
-                load(s + D, pre[u % D]);
-                step_barrier();
-            }
+            const int b = sl & (NR - 1);
+            if (sl >= NR) wait_ge(&xcnt[b], 2 * NWY * (sl / NR)); // read by every XZ wave
+            ypass(pre[u], b);
+            signal(&ycnt[b]);
+            load(sl + D, pre[u]);
+        };
+        int s0 = 0;
+        for (; s0 + D <= nsl; s0 += D)
+            static_for<D>([&](auto ic) NDL_INLINE { yslice(ic, s0 + decltype(ic)::value); });
+        static_for<D>([&](auto ic) NDL_INLINE {
+            if (s0 + decltype(ic)::value < nsl) yslice(ic, s0 + decltype(ic)::value);
         });
     } else {
         // ------------------------------------------------------ XZ role
@@ -231,66 +256,63 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
         float ring[2][KZ];
 #pragma unroll
         for (int p = 0; p < KZ; p++) ring[0][p] = ring[1][p] = 0.f;
-        // step s x-filters slice j = s - 1 into ring slot j % KZ; once the
-        // ring holds KZ slices, the same step emits the z output of j.  KZ
-        // steps per block keep every slot a compile-time register; blocks
-        // whose steps all do both run without per-step checks.
-        auto step = [&](auto ic, int s, auto checked) NDL_INLINE {
-            constexpr int u = decltype(ic)::value;
-            constexpr int U = (u + KZ - 1) % KZ; // slot of slice s - 1 (blocks start at multiples of KZ)
-            constexpr bool CHK = decltype(checked)::value;
-            const int j = s - 1;
-            if (!CHK || (j >= 0 && j < nsl)) {
-                // x-pass: y-plane columns [c2, c2 + KX + 1) hold x in [c2 - cx, c2 - cx + KX + 1)
-                const float* yr = ycol + (j & 1) * PLANE;
-                float wv[NXW + (NXW & 1)];
+        // slice j: its x-pass into ring slot U = j % KZ, then (OUT) the z
+        // output of j.  Slices run in blocks of KZ, so every slot is a
+        // compile-time register; the first block fills the ring, full blocks
+        // run without per-slice checks, a checked block takes the rest.
+        auto xslice = [&](auto slot, int j, auto out) NDL_INLINE {
+            constexpr int U = decltype(slot)::value;
+            const int b = j & (NR - 1);
+            wait_ge(&ycnt[b], NWY * (j / NR + 1)); // written by every Y wave
+            const float* yr = ycol + b * PLANE;
+            float wv[NXW + (NXW & 1)];
 #pragma unroll
-                for (int m = 0; m < (NXW + 1) / 2; m++) {
-                    const float2 v = *reinterpret_cast<const float2*>(yr + 2 * m);
-                    wv[2 * m] = v.x;
-                    wv[2 * m + 1] = v.y;
-                }
-                float x0 = 0.f, x1 = 0.f;
-#pragma unroll
-                for (int t = 0; t < KX; t++) {
-                    const float wt = a.wx[t];
-                    x0 = __builtin_fmaf(wv[t], wt, x0);
-                    x1 = __builtin_fmaf(wv[t + 1], wt, x1);
-                }
-                ring[0][U] = x0;
-                ring[1][U] = x1;
-                if (!CHK || j >= KZ - 1) {
-                    float acc0 = 0.f, acc1 = 0.f;
-#pragma unroll
-                    for (int l = KZ - 1; l >= 0; l--) {
-                        const float wl = a.wzlag[l];
-                        acc0 = __builtin_fmaf(ring[0][(U - l + KZ) % KZ], wl, acc0);
-                        acc1 = __builtin_fmaf(ring[1][(U - l + KZ) % KZ], wl, acc1);
-                    }
-                    const auto rd = __builtin_amdgcn_make_buffer_rsrc((void*)orow, (short)0, nrec, 0x00020000);
-                    slab_u2 v = {__float_as_uint(acc0), __float_as_uint(acc1)};
-                    __builtin_amdgcn_raw_buffer_store_b64(v, rd, voff, 0, 0);
-                    orow += a.dz; // outputs leave in slice order
-                }
+            for (int m = 0; m < (NXW + 1) / 2; m++) {
+                const float2 v = *reinterpret_cast<const float2*>(yr + 2 * m);
+                wv[2 * m] = v.x;
+                wv[2 * m + 1] = v.y;
             }
-            step_barrier();
+            float x0 = 0.f, x1 = 0.f;
+#pragma unroll
+            for (int t = 0; t < KX; t++) {
+                const float wt = a.wx[t];
+                x0 = __builtin_fmaf(wv[t], wt, x0);
+                x1 = __builtin_fmaf(wv[t + 1], wt, x1);
+            }
+            __asm__ volatile("" :: "v"(x0), "v"(x1));
+            signal(&xcnt[b]); // the window is in registers
+            ring[0][U] = x0;
+            ring[1][U] = x1;
+            if constexpr (decltype(out)::value) {
+                float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+                for (int l = KZ - 1; l >= 0; l--) {
+                    const float wl = a.wzlag[l];
+                    acc0 = __builtin_fmaf(ring[0][(U - l + KZ) % KZ], wl, acc0);
+                    acc1 = __builtin_fmaf(ring[1][(U - l + KZ) % KZ], wl, acc1);
+                }
+                const auto rd = __builtin_amdgcn_make_buffer_rsrc((void*)orow, (short)0, nrec, 0x00020000);
+                slab_u2 v = {__float_as_uint(acc0), __float_as_uint(acc1)};
+                __builtin_amdgcn_raw_buffer_store_b64(v, rd, voff, 0, 0);
+                orow += a.dz; // outputs leave in slice order
+            }
         };
-        for (int s0 = 0; s0 < NSP; s0 += KZ) {
-            if (s0 >= KZ && s0 + KZ - 1 <= nsl) {
-                static_for<KZ>([&](auto ic) NDL_INLINE { step(ic, s0 + decltype(ic)::value, std::false_type{}); });
-            } else {
-                static_for<KZ>([&](auto ic) NDL_INLINE {
-                    const int s = s0 + decltype(ic)::value;
-                    if (s < NSP) step(ic, s, std::true_type{});
-                });
-            }
-        }
+        static_for<KZ>([&](auto ic) NDL_INLINE {
+            constexpr int u = decltype(ic)::value;
+            xslice(IC<u>{}, u, std::integral_constant<bool, u == KZ - 1>{});
+        });
+        int j0 = KZ;
+        for (; j0 + KZ <= nsl; j0 += KZ)
+            static_for<KZ>([&](auto ic) NDL_INLINE { xslice(ic, j0 + decltype(ic)::value, std::true_type{}); });
+        static_for<KZ>([&](auto ic) NDL_INLINE {
+            const int j = j0 + decltype(ic)::value;
+            if (j < nsl) xslice(ic, j, std::true_type{});
+        });
     }
 }
 
 // threads per workgroup: NWY Y waves + 2 NWY XZ waves
 constexpr int kSlabThreads = 3 * kTXW;
-
 
 template <int KX, int KY, int KZ, int D>
 __global__ __launch_bounds__(kSlabThreads) void slab_kernel(const FusedArgs a)
