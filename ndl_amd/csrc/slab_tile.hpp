@@ -16,11 +16,12 @@
 //            the last kz x-filtered slices; once kz slices are in, one z
 //            output pair per slice, one 8-byte store (512 contiguous bytes
 //            per wave).
-// The y-planes form a ring of NR buffers in LDS.  The roles synchronise per
-// buffer with LDS counters instead of a workgroup barrier: Y waves count the
-// writes of each buffer, XZ waves its reads, and each side waits (bounded
-// spin) only for the buffer it needs, so a Y wave whose loads arrive late
-// stalls the XZ waves only once they have used up the NR - 1 planes ahead.
+// The y-planes form a ring of 8 buffers in LDS, two groups of 4 slices.
+// The roles synchronise once per group with LDS counters instead of a
+// workgroup barrier per slice: Y waves count the groups they wrote, XZ waves
+// the groups they read, and each side waits (bounded spin) only for the
+// group it needs, so a Y wave whose loads arrive late stalls the XZ waves
+// only once they have used up the group ahead.
 // Only the z halo of a chunk (kz - 1 slices) is filtered twice; with one
 // long chunk per workgroup that is the only recomputation of the pass.
 //
@@ -109,14 +110,16 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
     constexpr int YP = W::PITCH;
     constexpr int NXW = KX + 1;           // x window of two outputs
     constexpr int PLANE = TY * YP;        // floats per y-plane
-    constexpr int NR = 4;                 // y-planes in the LDS ring
+    // the roles synchronise once per group of P slices: NR = 2 P y-planes,
+    // group k in slot k % 2
+    constexpr int P = 4, G = 2, NR = G * P;
     static_assert(KX % 2 == 1 && KX <= 17, "odd x kernels up to 17 taps");
     static_assert((D - 1) * NRY <= 63, "loads in flight exceed the wait counter");
     __shared__ __align__(16) float ypl[NR * PLANE];
-    // per ring buffer: Y waves that wrote it / XZ waves that read it, summed
-    // over its uses (use g of buffer b is slice g * NR + b)
-    __shared__ int ycnt[NR];
-    __shared__ int xcnt[NR];
+    // per group slot: Y waves that wrote its planes / XZ waves that read
+    // them, summed over its uses (use u of slot g is group u * G + g)
+    __shared__ int ycnt[G];
+    __shared__ int xcnt[G];
 
     // tile -> (y tile, z chunk, batch), y fastest
     uint32_t q = a.fdy.div(tile);
@@ -140,8 +143,8 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
     const bool z_interior = z0 - a.cz >= 0 && z0 - a.cz + nsl <= a.Z;
-    if (threadIdx.x < NR) ycnt[threadIdx.x] = 0;
-    if (threadIdx.x < NR) xcnt[threadIdx.x] = 0;
+    if (threadIdx.x < G) ycnt[threadIdx.x] = 0;
+    if (threadIdx.x < G) xcnt[threadIdx.x] = 0;
     __syncthreads();
 
     if (w < NWY) {
@@ -223,14 +226,15 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
         float pre[D][NRY];
 #pragma unroll
         for (int u = 0; u < D; u++) load(u, pre[u]);
-        // slice sl: once every XZ wave has read the previous use of its
-        // buffer, the y-pass into it; then the loads of slice sl + D
+        // slice sl: at the start of a group, wait until every XZ wave has
+        // read the group's previous planes; the y-pass; at its end, count
+        // the group written; then the loads of slice sl + D
         auto yslice = [&](auto ic, int sl) NDL_INLINE {
             constexpr int u = decltype(ic)::value;
-            const int b = sl & (NR - 1);
-            if (sl >= NR) wait_ge(&xcnt[b], 2 * NWY * (sl / NR)); // read by every XZ wave
-            ypass(pre[u], b);
-            signal(&ycnt[b]);
+            const int k = sl / P; // group
+            if (sl % P == 0 && k >= G) wait_ge(&xcnt[k % G], 2 * NWY * (k / G)); // read by every XZ wave
+            ypass(pre[u], sl & (NR - 1));
+            if (sl % P == P - 1 || sl == nsl - 1) signal(&ycnt[k % G]);
             load(sl + D, pre[u]);
         };
         int s0 = 0;
@@ -262,9 +266,9 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
         // run without per-slice checks, a checked block takes the rest.
         auto xslice = [&](auto slot, int j, auto out) NDL_INLINE {
             constexpr int U = decltype(slot)::value;
-            const int b = j & (NR - 1);
-            wait_ge(&ycnt[b], NWY * (j / NR + 1)); // written by every Y wave
-            const float* yr = ycol + b * PLANE;
+            const int k = j / P; // group
+            if (j % P == 0) wait_ge(&ycnt[k % G], NWY * (k / G + 1)); // written by every Y wave
+            const float* yr = ycol + (j & (NR - 1)) * PLANE;
             float wv[NXW + (NXW & 1)];
 #pragma unroll
             for (int m = 0; m < (NXW + 1) / 2; m++) {
@@ -280,7 +284,7 @@ __device__ __forceinline__ void slab_tile(const FusedArgs& a, uint32_t tile)
                 x1 = __builtin_fmaf(wv[t + 1], wt, x1);
             }
             __asm__ volatile("" :: "v"(x0), "v"(x1));
-            signal(&xcnt[b]); // the window is in registers
+            if (j % P == P - 1 || j == nsl - 1) signal(&xcnt[k % G]); // the group's windows are in registers
             ring[0][U] = x0;
             ring[1][U] = x1;
             if constexpr (decltype(out)::value) {
