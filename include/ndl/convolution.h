@@ -284,10 +284,6 @@ void convolve_separable(const SrcImageT& src, DstImageT& dst, const std::array<s
     detail::run_sepconv(src, dst, plan, acc, NDL_ROUND_TRUNC);
 }
 
-/// Spatial gradient (processing/convolution.h:359-388): dst.slice(0, axis) is
-/// the central difference {-0.5, 0, +0.5} along `axis` (the zero tap is
-/// skipped, as in convolve()), for every axis; dst has a leading component
-/// axis of size DIM.
 /// gaussian_blur() of one image at several sigmas: levels[l] =
 /// gaussian_blur(src, sigma[l], border), bit-identical to separate calls
 /// (the scale stack detect_keypoints builds, feature_detection.h:101-109);
@@ -359,6 +355,39 @@ auto downsample(const ImageT& src, int factor, int channelAxis = 0, BorderMode b
     return result;
 }
 
+/// downsample(gaussian_blur(src, sigma, border), factor, channelAxis) with the
+/// blurred image in src's type (convolution.h:107-186 then :241-305): one
+/// step of a Gaussian pyramid, bit-identical to the two calls.  The last
+/// blur pass, its rounding and the block mean run in one kernel, so the
+/// full-resolution blurred image is never stored.  Returns a new OwnedImage.
+template <class ImageT>
+auto gaussian_downsample(const ImageT& src, double sigma, int factor, int channelAxis = 0,
+                         BorderMode border = BorderMode::Clamp)
+{
+    using T = typename ImageT::value_type;
+    static_assert(std::is_arithmetic_v<T>, "ndl::gaussian_downsample() requires an arithmetic value_type");
+    constexpr int DIM = detail::dim_of<ImageT>;
+    static_assert(DIM >= 2, "ndl::gaussian_downsample() needs DIM >= 2");
+    assert(sigma > 0 && factor > 1);
+    auto e = src.extent();
+    std::array<int, DIM> oe;
+    for (int i = 0; i < DIM; i++) oe[i] = i == channelAxis ? e[i] : (e[i] - 1) / factor + 1;
+    OwnedImage<T, DIM> result(oe);
+    std::array<double, DIM> sg;
+    sg.fill(sigma);
+    const int32_t rounding = std::is_integral_v<T> ? NDL_ROUND_NEAREST : NDL_ROUND_TRUNC;
+    detail::DeviceSource<ImageT> s(src);
+    detail::DeviceDest<OwnedImage<T, DIM>> o(result);
+    detail::check(ndl_gaussian_downsample(&s.t, &o.t, sg.data(), detail::border_code(border), factor, channelAxis,
+                                          NDL_ACCUM_REF, rounding, nullptr));
+    o.finish();
+    return result;
+}
+
+/// Spatial gradient (processing/convolution.h:359-388): dst.slice(0, axis) is
+/// the central difference {-0.5, 0, +0.5} along `axis` (the zero tap is
+/// skipped, as in convolve()), for every axis; dst has a leading component
+/// axis of size DIM.
 template <class SrcImageT, class DstImageT>
 void gradient(const SrcImageT& src, DstImageT& dst, BorderMode border = BorderMode::Reflect)
 {

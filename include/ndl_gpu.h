@@ -178,6 +178,22 @@ int ndl_dog(const ndl_tensor* levels, const ndl_tensor* dogs, int32_t nlevels, v
 int ndl_downsample(const ndl_tensor* src, const ndl_tensor* dst, int32_t factor,
                    int32_t channel_axis, void* stream);
 
+/* ---- blur then downsample --------------------------------------------------
+ * downsample(gaussian_blur(src, sigma, border), factor, channelAxis) with the
+ * blurred image held in dst's element type: the reduce step of a Gaussian
+ * pyramid, composed from convolution.h:107-186 and :241-305.  sigma has ndim
+ * entries; dimension d is blurred with gaussian_blur()'s taps for sigma[d]
+ * (passes in dimension order) unless sigma[d] <= 0; at least one is > 0.  The
+ * reference composition is every sigma[d] equal, NDL_ACCUM_REF and, for an
+ * integral dst, NDL_ROUND_NEAREST (gaussian_blur's final rounding); then the
+ * result is bit-identical to the two calls.  The last blurred pass, that
+ * rounding and the block mean run in one kernel: the full-resolution blurred
+ * image is never stored.  accum NDL_ACCUM_F32 needs f32/f16/bf16 src and dst.
+ * dst extents as for ndl_downsample; src and dst must not overlap. */
+int ndl_gaussian_downsample(const ndl_tensor* src, const ndl_tensor* dst, const double* sigma,
+                            int32_t border, int32_t factor, int32_t channel_axis, int32_t accum,
+                            int32_t rounding, void* stream);
+
 /* ---- adjoint (no counterpart in the reference) -------------------------
  * Gradient of L = <w, sepconv_forward(x)> with respect to x: grad_in =
  * A^T grad_out, the exact transpose of the forward operator including its

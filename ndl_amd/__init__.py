@@ -41,6 +41,8 @@ SIGNATURES = {
     "ndl_morph_separable": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
     "ndl_morph_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, C.c_int32, _VP]),
     "ndl_downsample": (C.c_int, [_T, _T, C.c_int32, C.c_int32, _VP]),
+    "ndl_gaussian_downsample": (C.c_int, [_T, _T, _DP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                          _VP]),
     "ndl_gaussian_scales": (C.c_int, [_T, _T, C.c_int32, _DP, C.c_int32, C.c_int32, C.c_int32, _VP]),
     "ndl_dog": (C.c_int, [_T, _T, C.c_int32, _VP]),
     "ndl_sepconv_adjoint_input": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
@@ -268,6 +270,21 @@ def downsample_dev(src, dst, factor, channel_axis=0, stream=None):
     return check(lib().ndl_downsample(C.byref(ts), C.byref(td), int(factor), int(channel_axis), stream))
 
 
+def gaussian_downsample_dev(src, dst, sigma, factor, channel_axis=0, border="clamp", accum=abi.ACCUM_REF,
+                            rounding=abi.ROUND_TRUNC, stream=None):
+    """downsample(gaussian_blur(src, sigma, border), factor, channel_axis) for
+    DeviceArrays; sigma is a scalar (every axis) or one value per ndl axis
+    (<= 0: not blurred)."""
+    nd = len(src.shape)
+    sg = np.full(nd, float(sigma)) if np.isscalar(sigma) else np.ascontiguousarray(sigma, dtype=np.float64)
+    if sg.shape != (nd,):
+        raise ValueError(f"sigma needs {nd} entries")
+    b = abi.BORDERS[border] if isinstance(border, str) else int(border)
+    ts, td = src.tensor(), dst.tensor()
+    return check(lib().ndl_gaussian_downsample(C.byref(ts), C.byref(td), sg.ctypes.data_as(_DP), b, int(factor),
+                                               int(channel_axis), accum, rounding, stream))
+
+
 def separable_mask(mask):
     """Per-axis inclusion masks (numpy order) when the nonzero set of `mask`
     is their product, else None."""
@@ -341,6 +358,18 @@ def downsample(img, factor, channel_axis=0):
     s = DeviceArray.from_numpy(img)
     d = DeviceArray(shape, img.dtype)
     downsample_dev(s, d, factor, channel_axis)
+    return d.to_numpy()
+
+
+def gaussian_downsample(img, sigma, factor, channel_axis=0, border="clamp"):
+    """downsample(gaussian_blur(img, sigma, border), factor, channel_axis) in
+    one call, bit-identical to the two (the blurred image in img's type)."""
+    n = img.ndim
+    shape = tuple(e if (n - 1 - i) == channel_axis else (e - 1) // factor + 1 for i, e in enumerate(img.shape))
+    rounding = abi.ROUND_NEAREST if (n >= 2 and np.issubdtype(img.dtype, np.integer)) else abi.ROUND_TRUNC
+    s = DeviceArray.from_numpy(img)
+    d = DeviceArray(shape, img.dtype)
+    gaussian_downsample_dev(s, d, sigma, factor, channel_axis, border, abi.ACCUM_REF, rounding)
     return d.to_numpy()
 
 

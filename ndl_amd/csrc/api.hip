@@ -1847,6 +1847,113 @@ NDL_API int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels,
     return NDL_OK;
 }
 
+// ------------------------------------------------------ blur then downsample
+// downsample(gaussian_blur(src, sigma, border), factor, channel_axis) with
+// the blurred image in dst's type (convolution.h:107-186, :241-305).  The
+// passes before the last blurred axis run as in the blur (FP64 intermediate
+// for NDL_ACCUM_REF); the last pass, the blur's final store and the block
+// mean run in one kernel, so the full-resolution blurred image is never
+// written or read back.
+NDL_API int ndl_gaussian_downsample(const ndl_tensor* src, const ndl_tensor* dst, const double* sigma, int32_t border,
+                                    int32_t factor, int32_t channel_axis, int32_t accum, int32_t rounding,
+                                    void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
+    const int nd = src->ndim;
+    if (nd < 2 || dst->ndim != nd) return fail(NDL_ERR_SHAPE, "blur-downsample needs ndim >= 2 and equal ndim");
+    if (!sigma) return fail(NDL_ERR_ARG, "null sigma");
+    if (border < NDL_CLAMP || border > NDL_REFLECT) return fail(NDL_ERR_ARG, "unknown border %d", border);
+    if (factor < 2) return fail(NDL_ERR_ARG, "factor %d < 2", factor);
+    if (channel_axis < 0 || channel_axis >= nd)
+        return fail(NDL_ERR_ARG, "channel axis %d outside 0..%d", channel_axis, nd - 1);
+    if (accum != NDL_ACCUM_REF && accum != NDL_ACCUM_F32) return fail(NDL_ERR_ARG, "unknown accumulation mode %d", accum);
+    if (rounding != NDL_ROUND_TRUNC && rounding != NDL_ROUND_NEAREST) return fail(NDL_ERR_ARG, "unknown rounding %d", rounding);
+    if (accum == NDL_ACCUM_F32 && !(is_f32_family(src->dtype) && is_f32_family(dst->dtype)))
+        return fail(NDL_ERR_DTYPE, "FP32 accumulation needs f32/f16/bf16 source and destination");
+    for (int d = 0; d < nd; d++) {
+        if (src->extent[d] < 1) return fail(NDL_ERR_SHAPE, "empty source extent on dimension %d", d);
+        int64_t want = d == channel_axis ? src->extent[d] : (src->extent[d] - 1) / factor + 1;
+        if (dst->extent[d] != want)
+            return fail(NDL_ERR_SHAPE, "dst extent %lld on dimension %d, expected %lld", (long long)dst->extent[d], d,
+                        (long long)want);
+    }
+    // the blurred axes, ascending, with gaussian_blur's taps
+    std::vector<std::vector<double>> taps;
+    std::vector<ndl_axis_kernel> k;
+    int32_t axes[NDL_MAX_DIM];
+    int nb = 0;
+    for (int d = 0; d < nd; d++) {
+        if (!(sigma[d] > 0)) continue;
+        if (!(3.0 * sigma[d] < 1e6)) return fail(NDL_ERR_ARG, "dimension %d: sigma too large", d);
+        taps.emplace_back(2 * (size_t)std::ceil(3.0 * sigma[d]) + 3);
+        int w = ndl_gaussian_taps(sigma[d], taps.back().data(), (int32_t)taps.back().size());
+        if (w < 0) return w;
+        taps.back().resize(w);
+        axes[nb++] = d;
+    }
+    if (nb == 0) return fail(NDL_ERR_ARG, "no dimension with sigma > 0");
+    for (int i = 0; i < nb; i++)
+        k.push_back(ndl_axis_kernel{taps[i].data(), (int32_t)taps[i].size(), (int32_t)taps[i].size() / 2, border});
+    if ((rc = check_axes(src, nb, axes, k.data()))) return rc;
+    if ((rc = check_no_overlap(src, dst))) return rc;
+    const int64_t n = numel(src), no = numel(dst);
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    hipStream_t s = (hipStream_t)stream;
+    // passes before the last blurred axis, into a dense accumulator-typed image
+    const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
+    Scratch midbuf;
+    ndl_tensor mid;
+    const ndl_tensor* in = src;
+    if (nb > 1) {
+        if ((rc = alloc_scratch(midbuf, (size_t)n * dtype_size(acc_dt), s))) return rc;
+        mid = dense_like(src, midbuf.p, acc_dt);
+        if ((rc = forward_impl(src, &mid, nb - 1, axes, k.data(), accum, NDL_ROUND_TRUNC, s))) return rc;
+        in = &mid;
+    }
+    BlurDownArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = in->data;
+    a.out = dst->data;
+    a.nd = nd;
+    a.channel_axis = channel_axis;
+    a.factor = factor;
+    const ndl_axis_kernel& lk = k[nb - 1];
+    a.axis = axes[nb - 1];
+    a.K = lk.count;
+    a.center = lk.center;
+    a.border = border;
+    a.rounding = rounding;
+    for (int d = 0; d < nd; d++) {
+        a.ext[d] = src->extent[d];
+        a.oext[d] = dst->extent[d];
+        a.ist[d] = in->stride[d];
+        a.ost[d] = dst->stride[d];
+        a.fdiv[d].init((uint32_t)dst->extent[d]);
+    }
+    a.total = (uint32_t)no;
+    Scratch wide; // released (stream-ordered) after the launch
+    if (lk.count <= kArgTaps) {
+        for (int t = 0; t < lk.count; t++) {
+            a.taps_d[t] = lk.taps[t];
+            a.taps_f[t] = (float)lk.taps[t];
+        }
+    } else {
+        const size_t db = (size_t)lk.count * sizeof(double), fb = (size_t)lk.count * sizeof(float);
+        if ((rc = alloc_scratch(wide, db + fb, s))) return rc;
+        std::vector<unsigned char> host(db + fb);
+        memcpy(host.data(), lk.taps, db);
+        float* hf = (float*)(host.data() + db);
+        for (int t = 0; t < lk.count; t++) hf[t] = (float)lk.taps[t];
+        if ((rc = stage_upload(wide.p, host.data(), db + fb, s))) return rc;
+        a.wide_d = (const double*)wide.p;
+        a.wide_f = (const float*)((unsigned char*)wide.p + db);
+    }
+    hipError_t e = launch_blur_downsample(a, in->dtype, dst->dtype, accum, s);
+    if (e != hipSuccess) return hip_fail(e, "blur-downsample launch");
+    return NDL_OK;
+}
+
 NDL_API int ndl_dog(const ndl_tensor* levels, const ndl_tensor* dogs, int32_t nlevels, void* stream)
 {
     int rc;

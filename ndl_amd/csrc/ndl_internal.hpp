@@ -139,6 +139,30 @@ struct DownArgs {
 };
 hipError_t launch_downsample(const DownArgs& a, int dt, hipStream_t s);
 
+// The last axis pass of a blur fused with the block mean: per output, each
+// source position of its block in block order (dimension 0 fastest) gets the
+// pass along `axis` (taps ascending, zero taps skipped), is cast to the
+// output type as the blur would store it (`rounding`), and is summed in FP64;
+// the mean is stored as downsample() stores it.
+struct BlurDownArgs {
+    const void* in;            // input of the last pass, strides ist
+    void* out;
+    int32_t nd, channel_axis, factor;
+    int32_t axis, K, center, border, rounding;
+    int64_t ext[NDL_MAX_DIM];  // full-resolution extents
+    int64_t oext[NDL_MAX_DIM]; // output extents
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM]; // by the output extents, dimension 0 first
+    uint32_t total;
+    double taps_d[kArgTaps];
+    float taps_f[kArgTaps];
+    const double* wide_d;      // K > kArgTaps: the taps on the device (else null)
+    const float* wide_f;
+};
+// accum NDL_ACCUM_REF: TIn any, FP64; NDL_ACCUM_F32: f32-family in and out
+hipError_t launch_blur_downsample(const BlurDownArgs& a, int in_dt, int out_dt, int accum, hipStream_t s);
+
 // ------------------------------------------- tap-gradient partial sums
 // partial[blk*K + t] = sum over the block's share of positions p of
 // g[p] * u[B(p_axis + t - c)] (FP64), later summed in block order.

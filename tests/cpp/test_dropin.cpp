@@ -419,6 +419,21 @@ static void downsample_tests()
     for (int y = 0; y < 6 && ok; y++)
         for (int x = 0; x < 4; x++) ok = ok && half(x, y) == (float)(y * 8 + 2 * x) + 0.5f;
     report("downsample() along x only (channelAxis 1) averages pairs", ok);
+    // one pyramid step in one call equals gaussian_blur() then downsample()
+    std::mt19937 rng(12);
+    OwnedImage<uint8_t, 3> colour({3, 45, 31});
+    for (auto it = colour.begin(); it != colour.end(); ++it) *it = (uint8_t)(rng() & 255);
+    OwnedImage<uint8_t, 3> blurred(colour.extent());
+    gaussian_blur(colour, blurred, 1.0);
+    auto twoCalls = downsample(blurred, 2);
+    auto oneCall = gaussian_downsample(colour, 1.0, 2);
+    report("gaussian_downsample() equals gaussian_blur() then downsample() (uint8 colour)", oneCall == twoCalls);
+    OwnedImage<float, 2> gray({37, 29});
+    for (auto it = gray.begin(); it != gray.end(); ++it) *it = std::uniform_real_distribution<float>(0, 1)(rng);
+    OwnedImage<float, 2> gb(gray.extent());
+    gaussian_blur(gray, gb, 1.7, BorderMode::Reflect);
+    report("gaussian_downsample() equals gaussian_blur() then downsample() (float, factor 3, reflect)",
+           gaussian_downsample(gray, 1.7, 3, 1, BorderMode::Reflect) == downsample(gb, 3, 1));
 }
 
 // the scale stack and DoG layers of detect_keypoints (feature_detection.h:91-120)
