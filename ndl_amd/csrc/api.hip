@@ -1929,9 +1929,34 @@ NDL_API int ndl_gaussian_downsample(const ndl_tensor* src, const ndl_tensor* dst
         a.oext[d] = dst->extent[d];
         a.ist[d] = in->stride[d];
         a.ost[d] = dst->stride[d];
-        a.fdiv[d].init((uint32_t)dst->extent[d]);
+        if (d == a.axis) continue;
+        a.ldim[a.nl] = d;
+        a.lfd[a.nl++].init((uint32_t)dst->extent[d]);
     }
-    a.total = (uint32_t)no;
+    a.lanes = (uint32_t)(no / dst->extent[a.axis]);
+    a.out_dt = dst->dtype;
+    // the march kernel: the accumulator-typed intermediate as input, a tap
+    // count it is built for with no zero tap, at most 64 positions per block
+    // across the other decimated dimensions, and the axis slowest in every
+    // block (no decimated dimension after it)
+    bool march = in->dtype == acc_dt && lk.count >= 3 && lk.count <= 25 && lk.count % 2 == 1 &&
+                 lk.center == lk.count / 2 && src->extent[a.axis] < 0x7fffffff;
+    for (int t = 0; t < lk.count; t++)
+        if (lk.taps[t] == 0.0) march = false;
+    a.P = 1;
+    for (int d = 0; d < nd && march; d++) {
+        if (d == a.axis || d == channel_axis) continue;
+        if (d > a.axis && a.axis != channel_axis && src->extent[d] > 1) march = false;
+        if ((a.P *= factor) > 64) march = false;
+    }
+    a.ncol = march ? 64 / a.P : 0;
+    // a chunk of outputs along the axis per lane, short enough that the
+    // launch has several workgroups per CU
+    const int64_t oz = dst->extent[a.axis];
+    const int64_t lane_blocks = march ? ((int64_t)a.lanes + 4 * a.ncol - 1) / (4 * a.ncol) : ((int64_t)a.lanes + 255) / 256;
+    const int64_t want_chunks = std::max<int64_t>(1, 8 * (int64_t)device_cus() / lane_blocks);
+    a.tz = (int32_t)std::max<int64_t>(1, std::min<int64_t>(oz, std::max<int64_t>(4, (oz + want_chunks - 1) / want_chunks)));
+    a.tiles_z = (int32_t)((oz + a.tz - 1) / a.tz);
     Scratch wide; // released (stream-ordered) after the launch
     if (lk.count <= kArgTaps) {
         for (int t = 0; t < lk.count; t++) {
@@ -1948,6 +1973,11 @@ NDL_API int ndl_gaussian_downsample(const ndl_tensor* src, const ndl_tensor* dst
         if ((rc = stage_upload(wide.p, host.data(), db + fb, s))) return rc;
         a.wide_d = (const double*)wide.p;
         a.wide_f = (const float*)((unsigned char*)wide.p + db);
+    }
+    if (march && !env_int("NDL_DISABLE_BLUR_DOWN_MARCH", 0)) {
+        hipError_t e = launch_blur_down_march(a, accum, s);
+        if (e == hipSuccess) return NDL_OK;
+        if (e != hipErrorInvalidValue) return hip_fail(e, "blur-downsample launch");
     }
     hipError_t e = launch_blur_downsample(a, in->dtype, dst->dtype, accum, s);
     if (e != hipSuccess) return hip_fail(e, "blur-downsample launch");

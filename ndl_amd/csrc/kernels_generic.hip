@@ -537,67 +537,81 @@ template <> __device__ __forceinline__ float bd_tap<float>(const BlurDownArgs& a
     return a.wide_f ? a.wide_f[t] : a.taps_f[t];
 }
 
+// Each lane marches one column of outputs along the pass axis (a chunk of
+// tz of them), so the inputs of consecutive outputs, which overlap by all but
+// `factor` slices, are re-read from the lane's own recent loads in cache.
 template <class TIn, class TOut, class Acc>
 __global__ __launch_bounds__(256) void blur_downsample_kernel(const BlurDownArgs a)
 {
-    uint32_t idx = blockIdx.x * 256u + threadIdx.x;
-    if (idx >= a.total) return;
-    int64_t lo[NDL_MAX_DIM], hi[NDL_MAX_DIM], cur[NDL_MAX_DIM];
-    int64_t ooff = 0, count = 1;
-    uint32_t rem = idx;
-    for (int d = 0; d < a.nd; d++) {
-        uint32_t q = a.fdiv[d].div(rem);
-        const int64_t c = (int64_t)(rem - q * (uint32_t)a.oext[d]);
+    const uint32_t chunk = blockIdx.x % (uint32_t)a.tiles_z;
+    const uint32_t lane = blockIdx.x / (uint32_t)a.tiles_z * 256u + threadIdx.x;
+    if (lane >= a.lanes) return;
+    int64_t oc[NDL_MAX_DIM];
+    uint32_t rem = lane;
+    for (int i = 0; i < a.nl; i++) {
+        const int d = a.ldim[i];
+        const uint32_t q = a.lfd[i].div(rem);
+        oc[d] = (int64_t)(rem - q * (uint32_t)a.oext[d]);
         rem = q;
-        ooff += c * a.ost[d];
-        if (d == a.channel_axis) {
-            lo[d] = c;
-            hi[d] = c + 1;
-        } else {
-            lo[d] = c * a.factor;
-            hi[d] = min(lo[d] + a.factor, a.ext[d]);
-        }
-        cur[d] = lo[d];
-        count *= hi[d] - lo[d];
     }
     const int64_t m = a.ext[a.axis], as = a.ist[a.axis];
-    double sum = 0.0;
-    for (int64_t n = 0; n < count; n++) {
-        int64_t off = 0;
-        for (int d = 0; d < a.nd; d++)
-            if (d != a.axis) off += cur[d] * a.ist[d];
-        const TIn* line = (const TIn*)a.in + off;
-        const int64_t j = cur[a.axis];
-        Acc total = 0;
-        for (int t = 0; t < a.K; t++) {
-            const Acc w = bd_tap<Acc>(a, t);
-            if (w == (Acc)0) continue;
-            int64_t x = j + t - a.center;
-            if (x < 0 || x >= m) x = border_map(a.border, m, x);
-            total = mac<Acc>(total, load_acc<Acc>(line + x * as), w);
-        }
-        TOut blurred; // the value gaussian_blur would have stored
-        store_acc<TOut, Acc>(&blurred, total, a.rounding);
-        sum = sum + load_f64(&blurred);
-        for (int d = 0; d < a.nd; d++) { // block odometer, dimension 0 fastest
-            if (d == a.channel_axis) continue;
-            if (++cur[d] < hi[d]) break;
+    const int64_t oz0 = (int64_t)chunk * a.tz, oz1 = min(oz0 + a.tz, a.oext[a.axis]);
+    for (int64_t oz = oz0; oz < oz1; oz++) {
+        oc[a.axis] = oz;
+        int64_t lo[NDL_MAX_DIM], hi[NDL_MAX_DIM], cur[NDL_MAX_DIM];
+        int64_t ooff = 0, count = 1;
+        for (int d = 0; d < a.nd; d++) {
+            const int64_t c = oc[d];
+            ooff += c * a.ost[d];
+            if (d == a.channel_axis) {
+                lo[d] = c;
+                hi[d] = c + 1;
+            } else {
+                lo[d] = c * a.factor;
+                hi[d] = min(lo[d] + a.factor, a.ext[d]);
+            }
             cur[d] = lo[d];
+            count *= hi[d] - lo[d];
         }
+        double sum = 0.0;
+        for (int64_t n = 0; n < count; n++) {
+            int64_t off = 0;
+            for (int d = 0; d < a.nd; d++)
+                if (d != a.axis) off += cur[d] * a.ist[d];
+            const TIn* line = (const TIn*)a.in + off;
+            const int64_t j = cur[a.axis];
+            Acc total = 0;
+            for (int t = 0; t < a.K; t++) {
+                const Acc w = bd_tap<Acc>(a, t);
+                if (w == (Acc)0) continue;
+                int64_t x = j + t - a.center;
+                if (x < 0 || x >= m) x = border_map(a.border, m, x);
+                total = mac<Acc>(total, load_acc<Acc>(line + x * as), w);
+            }
+            TOut blurred; // the value gaussian_blur would have stored
+            store_acc<TOut, Acc>(&blurred, total, a.rounding);
+            sum = sum + load_f64(&blurred);
+            for (int d = 0; d < a.nd; d++) { // block odometer, dimension 0 fastest
+                if (d == a.channel_axis) continue;
+                if (++cur[d] < hi[d]) break;
+                cur[d] = lo[d];
+            }
+        }
+        store_from_f64((TOut*)a.out + ooff, sum / (double)count, NDL_ROUND_NEAREST);
     }
-    store_from_f64((TOut*)a.out + ooff, sum / (double)count, NDL_ROUND_NEAREST);
 }
 
 hipError_t launch_blur_downsample(const BlurDownArgs& a, int in_dt, int out_dt, int accum, hipStream_t s)
 {
-    if (a.total == 0) return hipSuccess;
-    uint32_t blocks = (a.total + 255u) / 256u;
+    const uint64_t blocks = (uint64_t)a.tiles_z * ((a.lanes + 255u) / 256u);
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0xffffffffull) return hipErrorInvalidValue;
     auto go = [&](auto acc_tag, auto tin) {
         using Acc = decltype(acc_tag);
         using TIn = decltype(tin);
         return with_dtype(out_dt, [&](auto tout) {
             using TOut = decltype(tout);
-            hipLaunchKernelGGL((blur_downsample_kernel<TIn, TOut, Acc>), dim3(blocks), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((blur_downsample_kernel<TIn, TOut, Acc>), dim3((uint32_t)blocks), dim3(256), 0, s, a);
             return hipGetLastError();
         });
     };

@@ -373,7 +373,165 @@ template <class F> hipError_t with_row_instance(const LineKey& key, F&& f)
     });
 }
 
+// ------------------------------------------------- blur then downsample
+// A blurred value as gaussian_blur would store it in the output type (code
+// dt), read back as double; and the block mean stored as downsample() does.
+template <class Acc> __device__ __forceinline__ double stored_value(Acc v, int dt, int rounding)
+{
+    auto rt = [&](auto tag) {
+        using T = decltype(tag);
+        T b;
+        if constexpr (sizeof(Acc) == 8) store_from_f64(&b, v, rounding);
+        else store_from_f32(&b, v);
+        return load_f64(&b);
+    };
+    switch (dt) {
+    case NDL_U8: return rt(uint8_t{});
+    case NDL_I8: return rt(int8_t{});
+    case NDL_U16: return rt(uint16_t{});
+    case NDL_I16: return rt(int16_t{});
+    case NDL_I32: return rt(int32_t{});
+    case NDL_F32: return rt(float{});
+    case NDL_F16: return rt(f16_t{});
+    case NDL_BF16: return rt(bf16_t{});
+    default: return (double)v;
+    }
+}
+
+__device__ __forceinline__ void store_mean(void* out, int64_t off, double v, int dt)
+{
+    switch (dt) {
+    case NDL_U8: store_from_f64((uint8_t*)out + off, v, NDL_ROUND_NEAREST); break;
+    case NDL_I8: store_from_f64((int8_t*)out + off, v, NDL_ROUND_NEAREST); break;
+    case NDL_U16: store_from_f64((uint16_t*)out + off, v, NDL_ROUND_NEAREST); break;
+    case NDL_I16: store_from_f64((int16_t*)out + off, v, NDL_ROUND_NEAREST); break;
+    case NDL_I32: store_from_f64((int32_t*)out + off, v, NDL_ROUND_NEAREST); break;
+    case NDL_F32: store_from_f64((float*)out + off, v, NDL_ROUND_NEAREST); break;
+    case NDL_F16: store_from_f64((f16_t*)out + off, v, NDL_ROUND_NEAREST); break;
+    case NDL_BF16: store_from_f64((bf16_t*)out + off, v, NDL_ROUND_NEAREST); break;
+    default: store_from_f64((double*)out + off, v, NDL_ROUND_NEAREST); break;
+    }
+}
+
+template <class Acc> __device__ __forceinline__ Acc bd_tap(const BlurDownArgs& a, int t);
+template <> __device__ __forceinline__ double bd_tap<double>(const BlurDownArgs& a, int t) { return a.taps_d[t]; }
+template <> __device__ __forceinline__ float bd_tap<float>(const BlurDownArgs& a, int t) { return a.taps_f[t]; }
+
+// Lanes of a wave: position in the block p slowest, output column fastest
+// (ncol columns, P positions: p * ncol + column), so each position reads
+// ncol consecutive columns.  Every lane marches its source column along the
+// axis with a ring of K + D registers (as line_march_kernel), rounds each
+// pass value into the output type, and the p = 0 lane of each column adds
+// the P values of a slice in position order (the other decimated dimensions,
+// dimension 0 fastest), slice after slice: downsample()'s odometer with the
+// axis slowest.  At a block's last slice it stores the mean.  Lanes past P,
+// past the columns or on a position beyond the ragged edge read column 0
+// and add nothing.
+template <int K, class Acc>
+__global__ __launch_bounds__(256) void blur_down_march_kernel(const BlurDownArgs a)
+{
+    constexpr int D = 8, R = K + D;
+    const uint32_t chunk = blockIdx.x % (uint32_t)a.tiles_z;
+    const uint32_t wave = blockIdx.x / (uint32_t)a.tiles_z * 4u + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int p = lane / a.ncol;
+    const int ci = lane - p * a.ncol;
+    const uint32_t col = wave * (uint32_t)a.ncol + (uint32_t)ci;
+    bool valid = p < a.P && col < a.lanes;
+    int64_t so = 0, dof = 0;
+    int pcount = 1; // positions of this column's block across the other dimensions
+    {
+        uint32_t rem = col < a.lanes ? col : 0;
+        int pr = p;
+        for (int i = 0; i < a.nl; i++) {
+            const int d = a.ldim[i];
+            const uint32_t q = a.lfd[i].div(rem);
+            const int64_t oc = (int64_t)(rem - q * (uint32_t)a.oext[d]);
+            rem = q;
+            dof += oc * a.ost[d];
+            if (d == a.channel_axis) {
+                so += oc * a.ist[d];
+                continue;
+            }
+            const int dig = pr % a.factor;
+            pr /= a.factor;
+            const int64_t sc = oc * a.factor + dig;
+            pcount *= (int)min((int64_t)a.factor, a.ext[d] - oc * a.factor);
+            if (sc >= a.ext[d]) valid = false;
+            so += sc * a.ist[d];
+        }
+    }
+    if (!valid) so = 0;
+    const int Z = (int)a.ext[a.axis];
+    const int f = a.axis == a.channel_axis ? 1 : a.factor; // slices per block
+    const int ob0 = (int)chunk * a.tz;
+    const int ob1 = min(ob0 + a.tz, (int)a.oext[a.axis]);
+    const int zs0 = ob0 * f;
+    const int n_src = min(ob1 * f, Z) - zs0;
+    const int n_in = n_src + K - 1;
+    const int p0 = zs0 - a.center;
+    const Acc* scol = (const Acc*)a.in + so;
+    const int64_t sz = a.ist[a.axis], dz = a.ost[a.axis];
+    const int src_lane = ci;
+
+    Acc ring[R];
+    auto load = [&](int i) { return scol[(int64_t)safe_map32(a.border, Z, p0 + i) * sz]; };
+#pragma unroll
+    for (int j = 0; j < D; j++)
+        if (j < n_in) ring[j] = load(j);
+    double sum = 0.0;
+    int nz = 0, ob = ob0;
+    for (int i0 = 0; i0 < n_in; i0 += R) {
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            const int i = i0 + u;
+            if (i + D < n_in) ring[(u + D) % R] = load(i + D);
+            if (i >= K - 1 && i < n_in) {
+                Acc acc = 0;
+#pragma unroll
+                for (int t = 0; t < K; t++) acc = mac1<Acc>(acc, bd_tap<Acc>(a, t), ring[(u - (K - 1) + t + R) % R]);
+                const double v = stored_value<Acc>(acc, a.out_dt, a.rounding);
+                const int ok = valid ? 1 : 0;
+                for (int q = 0; q < a.P; q++) {
+                    const double vq = __shfl(v, src_lane + q * a.ncol);
+                    if (__shfl(ok, src_lane + q * a.ncol)) sum = sum + vq;
+                }
+                nz++;
+                if (nz == f || zs0 + i - (K - 1) == Z - 1) {
+                    if (p == 0 && valid) store_mean(a.out, dof + (int64_t)ob * dz, sum / (double)(pcount * nz), a.out_dt);
+                    sum = 0.0;
+                    nz = 0;
+                    ob++;
+                }
+            }
+        }
+    }
+}
+
+template <class F> hipError_t with_blur_down_instance(int K, int accum, F&& f)
+{
+    return with_k(K, [&](auto kk) {
+        if (accum == NDL_ACCUM_REF) return f(kk, double{});
+        if (accum == NDL_ACCUM_F32) return f(kk, float{});
+        return hipErrorInvalidValue;
+    });
+}
+
 } // namespace
+
+hipError_t launch_blur_down_march(const BlurDownArgs& a, int accum, hipStream_t s)
+{
+    if (a.P < 1 || a.P > 64 || a.ncol != 64 / a.P) return hipErrorInvalidValue;
+    const uint64_t waves = ((uint64_t)a.lanes + a.ncol - 1) / a.ncol;
+    const uint64_t blocks = (uint64_t)a.tiles_z * ((waves + 3) / 4);
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0xffffffffull) return hipErrorInvalidValue;
+    return with_blur_down_instance(a.K, accum, [&](auto kk, auto ac) {
+        hipLaunchKernelGGL((blur_down_march_kernel<decltype(kk)::value, decltype(ac)>), dim3((uint32_t)blocks),
+                           dim3(256), 0, s, a);
+        return hipGetLastError();
+    });
+}
 
 bool row_pass_supported(const LineKey& key)
 {

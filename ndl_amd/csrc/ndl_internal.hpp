@@ -153,15 +153,30 @@ struct BlurDownArgs {
     int64_t oext[NDL_MAX_DIM]; // output extents
     int64_t ist[NDL_MAX_DIM];
     int64_t ost[NDL_MAX_DIM];
-    FastDiv fdiv[NDL_MAX_DIM]; // by the output extents, dimension 0 first
-    uint32_t total;
+    // work item: one column of outputs along `axis` (the other output
+    // coordinates, decoded over ldim[0..nl-1], fastest first) and a chunk of
+    // tz outputs along it; lanes consecutive in the column index
+    int32_t nl;
+    int32_t ldim[NDL_MAX_DIM];
+    FastDiv lfd[NDL_MAX_DIM];
+    uint32_t lanes;
+    int32_t tz, tiles_z;
     double taps_d[kArgTaps];
     float taps_f[kArgTaps];
     const double* wide_d;      // K > kArgTaps: the taps on the device (else null)
     const float* wide_f;
+    // march kernel: positions per block across the other decimated
+    // dimensions (P <= 64), output columns per wave (64 / P), output type
+    int32_t P, ncol, out_dt;
 };
 // accum NDL_ACCUM_REF: TIn any, FP64; NDL_ACCUM_F32: f32-family in and out
 hipError_t launch_blur_downsample(const BlurDownArgs& a, int in_dt, int out_dt, int accum, hipStream_t s);
+// The same result when `axis` is the slowest dimension of every block (each
+// decimated dimension other than it comes earlier), the input is the
+// accumulator type (FP64 for NDL_ACCUM_REF, FP32 for NDL_ACCUM_F32), K is odd
+// in 3..25 with no zero tap and P <= 64: lanes march along the axis with a
+// register ring (kernels_line.hip).  hipErrorInvalidValue: no instance.
+hipError_t launch_blur_down_march(const BlurDownArgs& a, int accum, hipStream_t s);
 
 // ------------------------------------------- tap-gradient partial sums
 // partial[blk*K + t] = sum over the block's share of positions p of

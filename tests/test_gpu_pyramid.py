@@ -9,7 +9,8 @@ images with the channel axis first, 2-D and 3-D images, factors 2..4 with
 ragged last blocks, every border, per-axis sigmas (an unblurred channel axis,
 a single blurred axis read straight from the source), wide kernels (taps
 beyond those carried in kernel arguments) and strided sources.  Each case
-also matches the two separate GPU calls bit for bit.
+also matches the two separate GPU calls bit for bit, and the march kernel
+(lanes marching along the last blurred axis) matches the generic one.
 """
 import sys
 
@@ -159,3 +160,17 @@ def test_argument_errors():
         ndl_amd.gaussian_downsample_dev(x, good, 1.0, 2, 0, accum=F32)  # FP32 on uint8
     with pytest.raises(ndl_amd.NdlError):
         ndl_amd.gaussian_downsample_dev(x, good, 6.0, 2, 0, "reflect")  # taps reach past 12
+
+
+@pytest.mark.parametrize("case", CASES[:4] + CASES[6:], ids=lambda c: f"{c[0]}-f{c[2]}-ca{c[3]}")
+def test_generic_kernel_matches_the_march(case, monkeypatch):
+    # the cases above run the march kernel (lanes marching along the last
+    # blurred axis); the generic one-output-per-thread kernel must agree
+    shape, sigma, f, ca, border = case
+    rng = np.random.default_rng(66)
+    x = data(np.uint8, shape, rng)
+    a = ndl_amd.gaussian_downsample(x, sigma, f, ca, border)
+    monkeypatch.setenv("NDL_DISABLE_BLUR_DOWN_MARCH", "1")
+    b = ndl_amd.gaussian_downsample(x, sigma, f, ca, border)
+    assert same_bits(a, b)
+    assert same_bits(a, oracle.downsample(oracle.gaussian_blur(x, sigma, border), f, ca))
