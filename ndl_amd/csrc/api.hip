@@ -295,31 +295,118 @@ struct MorphCorner {
 int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
                  hipStream_t s, int op = 0, const MorphCorner* mc = nullptr);
 
-// per-axis chain through dense accumulator-typed intermediates
+// Two consecutive reference passes, along dimension ax then ay, in one
+// launch (kernels_xyref.hip); -1 when the kernel does not apply: unequal or
+// even tap counts or counts above 25, a zero tap, an x extent under 16 or a
+// y extent under 8 (mostly idle tiles), element types other than uint8,
+// uint16, int16, float32 and float64 (the output FP64 or the input's type),
+// or more than 2^31 tiles.
+int env_int(const char* name, int dflt);
+int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, const ndl_axis_kernel& kx,
+               const ndl_axis_kernel& ky, int rounding, hipStream_t s)
+{
+    if (!env_int("NDL_XY_REF", 1) || ax == ay) return -1;
+    if (kx.count != ky.count || kx.count > kXYMaxTaps || kx.count % 2 == 0) return -1;
+    for (int t = 0; t < kx.count; t++)
+        if (kx.taps[t] == 0.0 || ky.taps[t] == 0.0) return -1;
+    if (in->extent[ax] < 16 || in->extent[ay] < 8) return -1;
+    if (in->extent[ax] > 0x7fffffffLL || in->extent[ay] > 0x7fffffffLL) return -1;
+    const int it = in->dtype, ot = out->dtype;
+    if (it != NDL_U8 && it != NDL_U16 && it != NDL_I16 && it != NDL_F32 && it != NDL_F64) return -1;
+    if (ot != NDL_F64 && ot != it) return -1;
+    if (xy_ref_lds_bytes(kx.count, ky.count, (int)dtype_size(it)) > 65536) return -1;
+    XYRefArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = in->data;
+    a.out = out->data;
+    a.X = (int32_t)in->extent[ax];
+    a.Y = (int32_t)in->extent[ay];
+    a.isx = in->stride[ax];
+    a.isy = in->stride[ay];
+    a.osx = out->stride[ax];
+    a.osy = out->stride[ay];
+    a.kx = kx.count;
+    a.cx = kx.center;
+    a.bx = kx.border;
+    a.ky = ky.count;
+    a.cy = ky.center;
+    a.by = ky.border;
+    for (int t = 0; t < kx.count; t++) a.wx[t] = kx.taps[t];
+    for (int t = 0; t < ky.count; t++) a.wy[t] = ky.taps[t];
+    a.rounding = rounding;
+    a.tiles_x = (a.X + kXYTileX - 1) / kXYTileX;
+    a.tiles_y = (a.Y + kXYTileY - 1) / kXYTileY;
+    a.ftx.init((uint32_t)a.tiles_x);
+    a.fty.init((uint32_t)a.tiles_y);
+    int64_t tiles = (int64_t)a.tiles_x * a.tiles_y;
+    for (int d = 0; d < in->ndim; d++) {
+        if (d == ax || d == ay) continue;
+        a.bext[a.nb].init((uint32_t)std::min<int64_t>(in->extent[d], 0xffffffffLL));
+        a.ib[a.nb] = in->stride[d];
+        a.ob[a.nb++] = out->stride[d];
+        tiles *= in->extent[d];
+        if (tiles > 0x7fffffffLL) return -1;
+    }
+    if (tiles == 0) return NDL_OK;
+    a.ntiles = (uint32_t)tiles;
+    hipError_t e = launch_xy_ref(a, it, ot, s);
+    if (e == hipErrorInvalidValue) return -1;
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "two-pass reference launch");
+}
+
+// per-axis chain through dense accumulator-typed intermediates; in
+// reference mode consecutive pairs of passes run as one launch where the
+// two-pass kernel applies
 int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, const int32_t* axes,
                       const ndl_axis_kernel* k, int accum, int rounding, hipStream_t s)
 {
     const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
     Scratch buf[2];
     ndl_tensor mid[2];
-    if (naxes > 1) {
-        size_t bytes = (size_t)numel(src) * dtype_size(acc_dt);
-        for (int i = 0; i < 2 && i < naxes - 1; i++) {
-            int rc = alloc_scratch(buf[i], bytes, s);
+    // intermediate `slot`, allocated at its first use
+    auto inter = [&](int slot, const ndl_tensor*& t) -> int {
+        if (!buf[slot].p) {
+            int rc = alloc_scratch(buf[slot], (size_t)numel(src) * dtype_size(acc_dt), s);
             if (rc) return rc;
-            mid[i] = dense_like(src, buf[i].p, acc_dt);
+            mid[slot] = dense_like(src, buf[slot].p, acc_dt);
         }
-    }
+        t = &mid[slot];
+        return NDL_OK;
+    };
     const ndl_tensor* cur = src;
-    for (int i = 0; i < naxes; i++) {
-        bool last = i == naxes - 1;
-        const ndl_tensor* out = last ? dst : &mid[i & 1];
+    int slot = 0;
+    for (int i = 0; i < naxes;) {
+        if (accum == NDL_ACCUM_REF && i + 1 < naxes) {
+            const bool last = i + 2 == naxes;
+            const ndl_tensor* out = dst;
+            if (!last) {
+                int rc = inter(slot, out);
+                if (rc) return rc;
+            }
+            const int rc = try_xy_ref(cur, out, axes[i], axes[i + 1], k[i], k[i + 1],
+                                      last ? rounding : NDL_ROUND_TRUNC, s);
+            if (rc != -1) {
+                if (rc) return rc;
+                cur = out;
+                slot ^= 1;
+                i += 2;
+                continue;
+            }
+        }
+        const bool last = i == naxes - 1;
+        const ndl_tensor* out = dst;
+        if (!last) {
+            int rc = inter(slot, out);
+            if (rc) return rc;
+        }
         int rc = try_one_axis(cur, out, axes[i], k[i], accum, s);
         if (rc == -1)
             rc = run_axis_pass(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, PASS_GATHER,
                                nullptr, 0, 0, s);
         if (rc) return rc;
         cur = out;
+        slot ^= 1;
+        i++;
     }
     return NDL_OK;
 }

@@ -1,0 +1,183 @@
+// kernels_xyref.hip -- two consecutive passes of gaussian_blur()'s FP64 chain
+// (processing/convolution.h:151-185: axis 0 into an FP64 buffer, the next
+// axis from it) in one launch, the intermediate kept in LDS instead of HBM.
+//
+// Arithmetic (this file is built with -ffp-contract=off): per output and
+// pass, total = 0, then total = total + double(x) * tap for the taps in
+// ascending order, every product and sum rounded separately (the reference
+// skips exactly-zero taps, detail.h:164-174; kernels with one are not sent
+// here); the x pass result is an FP64 value, as the reference's buffer
+// holds; the y pass result is rounded once into the output type.
+// Bit-identical to the two passes of the chain.
+//
+// A workgroup (4 waves) owns a 64 x 32 tile of outputs (x by y) of one index
+// of the other dimensions:
+//  1. the input window, (32 + K - 1) rows of (64 + K - 1) elements with the
+//     border folded per coordinate, is read into LDS in the input type (wave
+//     w takes rows w, w + 4, ...; a lane one element, two when the row is
+//     wider than 64);
+//  2. the x pass over every window row, four consecutive outputs per task
+//     from a register window of K + 3 values, into an FP64 LDS plane;
+//  3. the y pass, four consecutive rows of one column per task from a
+//     register window of K + 3 values of that plane, stored to the output.
+// The tap count K (odd, 3..25, the same on both axes, as every
+// gaussian_blur() has) is a template parameter: the windows are registers
+// and the tap loops carry no branch.
+#include "ndl_internal.hpp"
+
+namespace ndl {
+namespace {
+
+constexpr int TXW = kXYTileX, TYH = kXYTileY;
+
+template <class TIn, class TOut, int K> __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
+{
+    extern __shared__ __align__(16) unsigned char xy_lds[];
+    constexpr int WA = TXW + K - 1, HA = TYH + K - 1;
+    double* plane = (double*)xy_lds;                                 // HA x TXW, x pass results
+    TIn* win = (TIn*)(xy_lds + (size_t)HA * TXW * sizeof(double));   // HA x WA, input window
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+    uint32_t t = blockIdx.x;
+    uint32_t q = a.ftx.div(t);
+    const int x0 = (int)(t - q * (uint32_t)a.tiles_x) * TXW;
+    t = q;
+    q = a.fty.div(t);
+    const int y0 = (int)(t - q * (uint32_t)a.tiles_y) * TYH;
+    t = q;
+    int64_t ib = 0, ob = 0;
+    for (int i = 0; i < a.nb; i++) {
+        q = a.bext[i].div(t);
+        const uint32_t r = t - q * a.bext[i].d;
+        ib += (int64_t)r * a.ib[i];
+        ob += (int64_t)r * a.ob[i];
+        t = q;
+    }
+    const TIn* in = (const TIn*)a.in + ib;
+
+    // 1. input window: every load of the wave is issued before the first
+    //    LDS store (rows are compile-time register slots)
+    {
+        constexpr int RW = (HA + 3) / 4; // window rows per wave, at most
+        const int64_t xo0 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + lane) * a.isx;
+        const bool two = lane + 64 < WA;
+        const int64_t xo1 = two ? (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + lane + 64) * a.isx : 0;
+        TIn e0[RW], e1[RW];
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int r = wv + 4 * j;
+            if (r < HA) {
+                const TIn* row = in + (int64_t)safe_map32(a.by, a.Y, y0 - a.cy + r) * a.isy;
+                e0[j] = row[xo0];
+                if (two) e1[j] = row[xo1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int r = wv + 4 * j;
+            if (r < HA) {
+                win[r * WA + lane] = e0[j];
+                if (two) win[r * WA + lane + 64] = e1[j];
+            }
+        }
+    }
+    __syncthreads();
+
+    // 2. x pass: task (window row r, outputs 4s .. 4s+3)
+    for (int r = threadIdx.x >> 4; r < HA; r += 16) {
+        const int s = threadIdx.x & 15;
+        const TIn* src = win + r * WA + 4 * s;
+        double v[K + 3];
+#pragma unroll
+        for (int i = 0; i < K + 3; i++) v[i] = load_f64(src + i);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const double w = a.wx[k];
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) plane[r * TXW + 4 * s + j] = acc[j];
+    }
+    __syncthreads();
+
+    // 3. y pass: task (column lane, rows 4s .. 4s+3)
+    const int x = x0 + lane;
+    for (int s = wv; s < TYH / 4; s += 4) {
+        const double* src = plane + 4 * s * TXW + lane;
+        double v[K + 3];
+#pragma unroll
+        for (int i = 0; i < K + 3; i++) v[i] = src[i * TXW];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const double w = a.wy[k];
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+        }
+        if (x < a.X) {
+            TOut* col = (TOut*)a.out + ob + (int64_t)x * a.osx;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int y = y0 + 4 * s + j;
+                if (y < a.Y) store_from_f64(col + (int64_t)y * a.osy, acc[j], a.rounding);
+            }
+        }
+    }
+}
+
+template <int V> using IC = std::integral_constant<int, V>;
+
+// f(TIn, TOut, K): uint8 / uint16 / int16 / float32 / float64 input, the
+// output FP64 or the input's type, odd K 3..25
+template <class F> hipError_t with_xy_instance(int in_dt, int out_dt, int k, F&& f)
+{
+    auto pick_k = [&](auto tin, auto tout) -> hipError_t {
+        switch (k) {
+#define NDL_K(KK) case KK: return f(tin, tout, IC<KK>{});
+            NDL_K(3) NDL_K(5) NDL_K(7) NDL_K(9) NDL_K(11) NDL_K(13) NDL_K(15) NDL_K(17) NDL_K(19) NDL_K(21)
+            NDL_K(23) NDL_K(25)
+#undef NDL_K
+        }
+        return hipErrorInvalidValue;
+    };
+    auto pick = [&](auto tin) -> hipError_t {
+        if (out_dt == NDL_F64) return pick_k(tin, double{});
+        if (out_dt == in_dt) return pick_k(tin, tin);
+        return hipErrorInvalidValue;
+    };
+    switch (in_dt) {
+    case NDL_U8: return pick(uint8_t{});
+    case NDL_U16: return pick(uint16_t{});
+    case NDL_I16: return pick(int16_t{});
+    case NDL_F32: return pick(float{});
+    case NDL_F64: return pick(double{});
+    }
+    return hipErrorInvalidValue;
+}
+
+} // namespace
+
+size_t xy_ref_lds_bytes(int kx, int ky, int in_size)
+{
+    const size_t ha = (size_t)(TYH + ky - 1);
+    return ha * TXW * sizeof(double) + ha * (size_t)(TXW + kx - 1) * (size_t)in_size;
+}
+
+hipError_t launch_xy_ref(const XYRefArgs& a, int in_dt, int out_dt, hipStream_t s)
+{
+    if (a.ntiles == 0) return hipSuccess;
+    if (a.kx != a.ky) return hipErrorInvalidValue;
+    return with_xy_instance(in_dt, out_dt, a.kx, [&](auto tin, auto tout, auto kk) {
+        using TIn = decltype(tin);
+        const size_t lds = xy_ref_lds_bytes(a.kx, a.ky, (int)sizeof(TIn));
+        if (lds > 65536) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((xy_ref_kernel<TIn, decltype(tout), decltype(kk)::value>), dim3(a.ntiles), dim3(256),
+                           lds, s, a);
+        return hipGetLastError();
+    });
+}
+
+} // namespace ndl

@@ -178,6 +178,33 @@ hipError_t launch_blur_downsample(const BlurDownArgs& a, int in_dt, int out_dt, 
 // register ring (kernels_line.hip).  hipErrorInvalidValue: no instance.
 hipError_t launch_blur_down_march(const BlurDownArgs& a, int accum, hipStream_t s);
 
+// ------------------------------------------- two reference passes at once
+// Two consecutive passes of the NDL_ACCUM_REF chain (kernels_xyref.hip):
+// along x, then along y (two distinct dimensions, any strides), FP64 with
+// every product and sum rounded separately, taps ascending, zero taps
+// skipped; the first pass's FP64 results stay in LDS.  A workgroup owns a
+// 64 x 32 tile of outputs (x by y) of one index of the other dimensions.
+constexpr int kXYTileX = 64, kXYTileY = 32, kXYMaxTaps = 25;
+struct XYRefArgs {
+    const void* in;
+    void* out;
+    int32_t X, Y;               // extents of the two axes
+    int64_t isx, isy, osx, osy; // strides (elements)
+    int32_t kx, cx, bx, ky, cy, by;
+    int32_t rounding;           // into the output type (integral outputs)
+    int32_t tiles_x, tiles_y;
+    FastDiv ftx, fty;
+    int32_t nb;                 // other dimensions
+    FastDiv bext[NDL_MAX_DIM];
+    int64_t ib[NDL_MAX_DIM], ob[NDL_MAX_DIM];
+    uint32_t ntiles;
+    double wx[kXYMaxTaps], wy[kXYMaxTaps];
+};
+// LDS bytes of one workgroup for these tap counts and input type size
+size_t xy_ref_lds_bytes(int kx, int ky, int in_size);
+// out_dt: NDL_F64 or in_dt; hipErrorInvalidValue: no instance
+hipError_t launch_xy_ref(const XYRefArgs& a, int in_dt, int out_dt, hipStream_t s);
+
 // ------------------------------------------- tap-gradient partial sums
 // partial[blk*K + t] = sum over the block's share of positions p of
 // g[p] * u[B(p_axis + t - c)] (FP64), later summed in block order.
