@@ -138,8 +138,35 @@ struct Scratch {
     }
 };
 
+// The current device's default memory pool keeps up to NDL_POOL_KEEP_BYTES
+// (default 8 GiB) of freed scratch for reuse instead of returning it to the
+// system at every synchronisation, after which the next call would map its
+// intermediates afresh.
+void keep_pool_memory()
+{
+    static std::mutex mu;
+    static bool done[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[dev]) return;
+    done[dev] = true;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    const char* v = getenv("NDL_POOL_KEEP_BYTES");
+    uint64_t keep = v && *v ? strtoull(v, nullptr, 10) : (8ull << 30);
+    if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) (void)hipGetLastError();
+}
+
 int alloc_scratch(Scratch& sc, size_t bytes, hipStream_t s)
 {
+    keep_pool_memory();
     sc.s = s;
     if (bytes == 0) bytes = 256;
     hipError_t e = hipMallocAsync(&sc.p, bytes, s);
@@ -293,14 +320,14 @@ struct MorphCorner {
     const int32_t* cbord;
 };
 int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
-                 hipStream_t s, int op = 0, const MorphCorner* mc = nullptr);
+                 hipStream_t s, int op = 0, const MorphCorner* mc = nullptr, int rounding = NDL_ROUND_TRUNC);
 
 // Two consecutive reference passes, along dimension ax then ay, in one
 // launch (kernels_xyref.hip); -1 when the kernel does not apply: unequal or
 // even tap counts or counts above 25, a zero tap, an x extent under 16 or a
 // y extent under 8 (mostly idle tiles), element types other than uint8,
-// uint16, int16, float32 and float64 (the output FP64 or the input's type),
-// or more than 2^31 tiles.
+// uint16, int16, float32 and float64 (the output FP64 or the input's type,
+// any of them from FP64), or more than 2^31 tiles.
 int env_int(const char* name, int dflt);
 int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, const ndl_axis_kernel& kx,
                const ndl_axis_kernel& ky, int rounding, hipStream_t s)
@@ -312,8 +339,9 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
     if (in->extent[ax] < 16 || in->extent[ay] < 8) return -1;
     if (in->extent[ax] > 0x7fffffffLL || in->extent[ay] > 0x7fffffffLL) return -1;
     const int it = in->dtype, ot = out->dtype;
-    if (it != NDL_U8 && it != NDL_U16 && it != NDL_I16 && it != NDL_F32 && it != NDL_F64) return -1;
-    if (ot != NDL_F64 && ot != it) return -1;
+    auto listed = [](int t) { return t == NDL_U8 || t == NDL_U16 || t == NDL_I16 || t == NDL_F32 || t == NDL_F64; };
+    if (!listed(it) || !listed(ot)) return -1;
+    if (ot != NDL_F64 && ot != it && it != NDL_F64) return -1;
     if (xy_ref_lds_bytes(kx.count, ky.count, (int)dtype_size(it)) > 65536) return -1;
     XYRefArgs a;
     memset(&a, 0, sizeof a);
@@ -399,7 +427,7 @@ int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxe
             int rc = inter(slot, out);
             if (rc) return rc;
         }
-        int rc = try_one_axis(cur, out, axes[i], k[i], accum, s);
+        int rc = try_one_axis(cur, out, axes[i], k[i], accum, s, 0, nullptr, last ? rounding : NDL_ROUND_TRUNC);
         if (rc == -1)
             rc = run_axis_pass(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, PASS_GATHER,
                                nullptr, 0, 0, s);
@@ -719,7 +747,9 @@ bool one_axis_types(const ndl_tensor* in, const ndl_tensor* out, const ndl_axis_
     } else if (accum == NDL_ACCUM_F32) {
         // float, bf16 or f16 storage, the same type in and out
         if (it != ot || (it != NDL_F32 && it != NDL_BF16 && it != NDL_F16)) return false;
-    } else if (accum != NDL_ACCUM_REF || (it != NDL_F32 && it != NDL_F64) || (ot != NDL_F32 && ot != NDL_F64)) {
+    } else if (accum != NDL_ACCUM_REF || (it != NDL_F32 && it != NDL_F64) ||
+               !(ot == NDL_F32 || ot == NDL_F64 ||
+                 (it == NDL_F64 && (ot == NDL_U8 || ot == NDL_U16 || ot == NDL_I16)))) {
         return false;
     }
     for (int t = 0; t < ak.count; t++)
@@ -873,17 +903,19 @@ bool plan_row(const ndl_tensor* in, const ndl_tensor* out, int xd, const ndl_axi
 // -1 when neither applies, an error code otherwise.  op: 0 convolution,
 // 1 erode, 2 dilate.
 int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& ak, int accum,
-                 hipStream_t s, int op, const MorphCorner* mc)
+                 hipStream_t s, int op, const MorphCorner* mc, int rounding)
 {
     LineKey key;
     LineArgs la;
     if (plan_line(in, out, axis, ak, accum, la, key, op, mc)) {
+        la.rounding = rounding;
         hipError_t e = launch_line_march(la, key, s);
         return e == hipSuccess ? NDL_OK : hip_fail(e, "line march launch");
     }
     RowArgs ra;
     uint32_t blocks = 0;
     if (!mc && plan_row(in, out, axis, ak, accum, ra, key, blocks, op)) {
+        ra.rounding = rounding;
         hipError_t e = launch_row_pass(ra, key, blocks, s);
         return e == hipSuccess ? NDL_OK : hip_fail(e, "row pass launch");
     }
@@ -1102,7 +1134,7 @@ int forward_impl(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxes, co
         if (rc != -1) return rc;
     }
     if (naxes == 1) {
-        const int rc = try_one_axis(src, dst, axes[0], k[0], accum, s);
+        const int rc = try_one_axis(src, dst, axes[0], k[0], accum, s, 0, nullptr, rounding);
         if (rc != -1) return rc;
     }
     if (naxes == 3) {

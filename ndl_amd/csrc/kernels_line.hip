@@ -63,9 +63,9 @@ template <class Acc, class T> __device__ __forceinline__ Acc to_acc(const T& x)
     if constexpr (sizeof(Acc) == 8) return load_f64(&x);
     else return load_f32(&x);
 }
-template <class Acc, class T> __device__ __forceinline__ void from_acc(T& o, Acc v)
+template <class Acc, class T> __device__ __forceinline__ void from_acc(T& o, Acc v, int rounding)
 {
-    if constexpr (sizeof(Acc) == 8) store_from_f64(&o, v, NDL_ROUND_TRUNC);
+    if constexpr (sizeof(Acc) == 8) store_from_f64(&o, v, rounding);
     else store_from_f32(&o, v);
 }
 
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
                             acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t),
                                                to_acc<Acc>(ring[(u - (K - 1) + t + R) % R][e]));
 #pragma unroll
-                    for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e]);
+                    for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e], a.rounding);
                 } else {
 #pragma unroll
                     for (int e = 0; e < V; e++) {
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void row_pass_kernel(const RowArgs a)
                 for (int e = 0; e < V; e++)
                     acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t), to_acc<Acc>(wv_[e + t]));
 #pragma unroll
-            for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e]);
+            for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e], a.rounding);
         } else {
 #pragma unroll
             for (int e = 0; e < V; e++) {
@@ -307,6 +307,10 @@ template <class F> hipError_t with_types(const LineKey& key, F&& f)
     if (key.in_dt == NDL_F32 && key.out_dt == NDL_F64) return f(float{}, double{}, double{}, IC<0>{});
     if (key.in_dt == NDL_F64 && key.out_dt == NDL_F64) return f(double{}, double{}, double{}, IC<0>{});
     if (key.in_dt == NDL_F64 && key.out_dt == NDL_F32) return f(double{}, float{}, double{}, IC<0>{});
+    // the last pass of a chain into an integral image (rounded per a.rounding)
+    if (key.in_dt == NDL_F64 && key.out_dt == NDL_U8) return f(double{}, uint8_t{}, double{}, IC<0>{});
+    if (key.in_dt == NDL_F64 && key.out_dt == NDL_U16) return f(double{}, uint16_t{}, double{}, IC<0>{});
+    if (key.in_dt == NDL_F64 && key.out_dt == NDL_I16) return f(double{}, int16_t{}, double{}, IC<0>{});
     return hipErrorInvalidValue;
 }
 

@@ -131,7 +131,8 @@ template <class TIn, class TOut, int K> __global__ __launch_bounds__(256) void x
 template <int V> using IC = std::integral_constant<int, V>;
 
 // f(TIn, TOut, K): uint8 / uint16 / int16 / float32 / float64 input, the
-// output FP64 or the input's type, odd K 3..25
+// output FP64 or the input's type (from float64: also uint8 / uint16 /
+// int16 / float32), odd K 3..25
 template <class F> hipError_t with_xy_instance(int in_dt, int out_dt, int k, F&& f)
 {
     auto pick_k = [&](auto tin, auto tout) -> hipError_t {
@@ -153,7 +154,15 @@ template <class F> hipError_t with_xy_instance(int in_dt, int out_dt, int k, F&&
     case NDL_U16: return pick(uint16_t{});
     case NDL_I16: return pick(int16_t{});
     case NDL_F32: return pick(float{});
-    case NDL_F64: return pick(double{});
+    case NDL_F64:
+        // an FP64 intermediate into the chain's final image
+        switch (out_dt) {
+        case NDL_U8: return pick_k(double{}, uint8_t{});
+        case NDL_U16: return pick_k(double{}, uint16_t{});
+        case NDL_I16: return pick_k(double{}, int16_t{});
+        case NDL_F32: return pick_k(double{}, float{});
+        }
+        return pick(double{});
     }
     return hipErrorInvalidValue;
 }

@@ -336,6 +336,7 @@ struct LineArgs {
     int64_t db[NDL_MAX_DIM];
     float w[kArgTaps];       // NDL_ACCUM_F32
     double wd[kArgTaps];     // NDL_ACCUM_REF
+    int32_t rounding;        // into an integral output (NDL_ACCUM_REF)
     // last erode / dilate pass of a float image: the original source, whose
     // value at the structuring element's corner (per dimension: first-tap
     // offset, border) replaces the result when it is NaN; null otherwise
@@ -374,6 +375,7 @@ struct RowArgs {
     int64_t db[NDL_MAX_DIM];
     float w[kArgTaps];       // NDL_ACCUM_F32
     double wd[kArgTaps];     // NDL_ACCUM_REF
+    int32_t rounding;        // into an integral output (NDL_ACCUM_REF)
 };
 // key.d is unused for rows
 hipError_t launch_row_pass(const RowArgs& a, const LineKey& key, uint32_t blocks, hipStream_t s);

@@ -10,7 +10,9 @@ tiles, x extents at the 16 minimum, tap counts 3..25 (25 on float64 exceeds
 the LDS budget and takes the chain), off-centre kernels, kernels with zero
 taps or unequal counts (the chain), every border, batch dimensions, axis
 pairs other than (0, 1) and in reverse order, and strided (mirrored,
-stepped) views.
+stepped) views.  Chains that end in an integral image run their last pass
+from the FP64 intermediate (a colour image: the channel pass, then x and y
+here; a volume: x and y here, then the z line march), rounded to nearest.
 """
 import sys
 
@@ -65,7 +67,7 @@ def gauss(k):
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16, np.int32, np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(100, 130), (37, 70, 5)[::-1], (9, 33, 16), (3, 4, 40, 66)])
+@pytest.mark.parametrize("shape", [(100, 130), (37, 70, 5)[::-1], (9, 33, 16), (3, 4, 40, 66), (50, 70, 3)])
 def test_types_and_shapes(dtype, shape, monkeypatch):
     rng = np.random.default_rng(71)
     x = data(dtype, shape, rng)
