@@ -327,10 +327,12 @@ int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const nd
 // even tap counts or counts above 25, a zero tap, an x extent under 16 or a
 // y extent under 8 (mostly idle tiles), element types other than uint8,
 // uint16, int16, float32 and float64 (the output FP64 or the input's type,
-// any of them from FP64), or more than 2^31 tiles.
+// any of them from FP64), or more than 2^31 tiles.  fx / fy > 1: out is the
+// downsampled image (block means over fx x fy, see XYRefArgs); dry: only
+// whether the kernel applies (0) or not (-1).
 int env_int(const char* name, int dflt);
 int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, const ndl_axis_kernel& kx,
-               const ndl_axis_kernel& ky, int rounding, hipStream_t s)
+               const ndl_axis_kernel& ky, int rounding, hipStream_t s, int fx = 1, int fy = 1, bool dry = false)
 {
     if (!env_int("NDL_XY_REF", 1) || ax == ay) return -1;
     if (kx.count != ky.count || kx.count > kXYMaxTaps || kx.count % 2 == 0) return -1;
@@ -362,6 +364,8 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
     for (int t = 0; t < kx.count; t++) a.wx[t] = kx.taps[t];
     for (int t = 0; t < ky.count; t++) a.wy[t] = ky.taps[t];
     a.rounding = rounding;
+    a.fx = fx;
+    a.fy = fy;
     a.tiles_x = (a.X + kXYTileX - 1) / kXYTileX;
     a.tiles_y = (a.Y + kXYTileY - 1) / kXYTileY;
     a.ftx.init((uint32_t)a.tiles_x);
@@ -375,7 +379,7 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
         tiles *= in->extent[d];
         if (tiles > 0x7fffffffLL) return -1;
     }
-    if (tiles == 0) return NDL_OK;
+    if (dry || tiles == 0) return NDL_OK;
     a.ntiles = (uint32_t)tiles;
     hipError_t e = launch_xy_ref(a, it, ot, s);
     if (e == hipErrorInvalidValue) return -1;
@@ -2019,10 +2023,33 @@ NDL_API int ndl_gaussian_downsample(const ndl_tensor* src, const ndl_tensor* dst
     const int64_t n = numel(src), no = numel(dst);
     if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
     hipStream_t s = (hipStream_t)stream;
-    // passes before the last blurred axis, into a dense accumulator-typed image
     const int acc_dt = accum == NDL_ACCUM_REF ? NDL_F64 : NDL_F32;
     Scratch midbuf;
     ndl_tensor mid;
+    // reference arithmetic: the last two blurred passes and the block means
+    // in one launch (kernels_xyref.hip) when the two axes are the only
+    // decimated ones, with factors 2 or 4 and near-unit-stride rows
+    if (accum == NDL_ACCUM_REF && nb >= 2 && env_int("NDL_XY_DOWN", 1)) {
+        const int ax = axes[nb - 2], ay = axes[nb - 1];
+        const int fx = ax == channel_axis ? 1 : factor, fy = ay == channel_axis ? 1 : factor;
+        bool ok = ax < ay && (fx == 1 || fx == 2 || fx == 4) && (fy == 1 || fy == 2 || fy == 4);
+        for (int d = 0; d < nd; d++)
+            if (d != ax && d != ay && d != channel_axis && src->extent[d] > 1) ok = false;
+        ndl_tensor pin = nb > 2 ? dense_like(src, nullptr, NDL_F64) : *src;
+        if (ok && std::llabs(pin.stride[ax]) <= 4 &&
+            try_xy_ref(&pin, dst, ax, ay, k[nb - 2], k[nb - 1], rounding, nullptr, fx, fy, true) == 0) {
+            const ndl_tensor* pi = src;
+            if (nb > 2) {
+                if ((rc = alloc_scratch(midbuf, (size_t)n * sizeof(double), s))) return rc;
+                mid = dense_like(src, midbuf.p, NDL_F64);
+                if ((rc = forward_impl(src, &mid, nb - 2, axes, k.data(), accum, NDL_ROUND_TRUNC, s))) return rc;
+                pi = &mid;
+            }
+            rc = try_xy_ref(pi, dst, ax, ay, k[nb - 2], k[nb - 1], rounding, s, fx, fy);
+            return rc != -1 ? rc : fail(NDL_ERR_ARG, "no two-pass instance for an accepted blur-downsample");
+        }
+    }
+    // passes before the last blurred axis, into a dense accumulator-typed image
     const ndl_tensor* in = src;
     if (nb > 1) {
         if ((rc = alloc_scratch(midbuf, (size_t)n * dtype_size(acc_dt), s))) return rc;

@@ -15,11 +15,13 @@
 //  1. the input window, (32 + K - 1) rows of (64 + K - 1) elements with the
 //     border folded per coordinate, is read into LDS in the input type (wave
 //     w takes rows w, w + 4, ...; a lane one element, two when the row is
-//     wider than 64);
+//     wider than 64; every load of the wave issued before the first store);
 //  2. the x pass over every window row, four consecutive outputs per task
 //     from a register window of K + 3 values, into an FP64 LDS plane;
 //  3. the y pass, four consecutive rows of one column per task from a
-//     register window of K + 3 values of that plane, stored to the output.
+//     register window of K + 3 values of that plane, stored to the output,
+//     or (a blur followed by downsample(), fx / fy > 1) rounded as the blur
+//     stores it and reduced to block means in the downsample's order.
 // The tap count K (odd, 3..25, the same on both axes, as every
 // gaussian_blur() has) is a template parameter: the windows are registers
 // and the tap loops carry no branch.
@@ -30,7 +32,45 @@ namespace {
 
 constexpr int TXW = kXYTileX, TYH = kXYTileY;
 
-template <class TIn, class TOut, int K> __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
+// downsample()'s block means (convolution.h:270-300) of the y pass results
+// of one task: rows 4s .. 4s+3 of column `lane` hold acc[0..3]; blocks of FY
+// rows (FY divides 4) by fx columns (adjacent lanes), each value rounded as
+// gaussian_blur stores it, summed in block order (x fastest, then y) by the
+// block's first lane, which stores the mean.
+template <class TOut, int FY>
+__device__ __forceinline__ void store_block_means(const XYRefArgs& a, const double* acc, int x0, int ybase, int lane,
+                                                  int64_t ob)
+{
+    double vr[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        TOut b;
+        store_from_f64(&b, acc[j], a.rounding);
+        vr[j] = load_f64(&b);
+    }
+    const int fx = a.fx;
+    const int xb = lane - lane % fx;
+    const int nx = min(fx, a.X - (x0 + xb));
+#pragma unroll
+    for (int b = 0; b < 4; b += FY) {
+        double sum = 0.0;
+#pragma unroll
+        for (int j = b; j < b + FY; j++) {
+            const bool yok = ybase + j < a.Y;
+            for (int dx = 0; dx < fx; dx++) {
+                const double v = __shfl(vr[j], xb + dx);
+                if (yok && dx < nx) sum = sum + v;
+            }
+        }
+        const int ny = min(FY, a.Y - (ybase + b));
+        if (lane == xb && nx > 0 && ny > 0)
+            store_from_f64((TOut*)a.out + ob + (int64_t)((x0 + xb) / fx) * a.osx + (int64_t)((ybase + b) / FY) * a.osy,
+                           sum / (double)(nx * ny), NDL_ROUND_NEAREST);
+    }
+}
+
+template <class TIn, class TOut, int K, bool DEC>
+__global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
 {
     extern __shared__ __align__(16) unsigned char xy_lds[];
     constexpr int WA = TXW + K - 1, HA = TYH + K - 1;
@@ -117,7 +157,11 @@ template <class TIn, class TOut, int K> __global__ __launch_bounds__(256) void x
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
         }
-        if (x < a.X) {
+        if constexpr (DEC) {
+            if (a.fy == 1) store_block_means<TOut, 1>(a, acc, x0, y0 + 4 * s, lane, ob);
+            else if (a.fy == 2) store_block_means<TOut, 2>(a, acc, x0, y0 + 4 * s, lane, ob);
+            else store_block_means<TOut, 4>(a, acc, x0, y0 + 4 * s, lane, ob);
+        } else if (x < a.X) {
             TOut* col = (TOut*)a.out + ob + (int64_t)x * a.osx;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -183,8 +227,12 @@ hipError_t launch_xy_ref(const XYRefArgs& a, int in_dt, int out_dt, hipStream_t 
         using TIn = decltype(tin);
         const size_t lds = xy_ref_lds_bytes(a.kx, a.ky, (int)sizeof(TIn));
         if (lds > 65536) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((xy_ref_kernel<TIn, decltype(tout), decltype(kk)::value>), dim3(a.ntiles), dim3(256),
-                           lds, s, a);
+        if (a.fx * a.fy > 1)
+            hipLaunchKernelGGL((xy_ref_kernel<TIn, decltype(tout), decltype(kk)::value, true>), dim3(a.ntiles),
+                               dim3(256), lds, s, a);
+        else
+            hipLaunchKernelGGL((xy_ref_kernel<TIn, decltype(tout), decltype(kk)::value, false>), dim3(a.ntiles),
+                               dim3(256), lds, s, a);
         return hipGetLastError();
     });
 }

@@ -199,6 +199,10 @@ struct XYRefArgs {
     int64_t ib[NDL_MAX_DIM], ob[NDL_MAX_DIM];
     uint32_t ntiles;
     double wx[kXYMaxTaps], wy[kXYMaxTaps];
+    // fx * fy > 1: store downsample()'s block means of the y pass results
+    // (rounded as the blur stores them) over fx x fy blocks (1, 2 or 4 each),
+    // at output strides osx / osy of the decimated image
+    int32_t fx, fy;
 };
 // LDS bytes of one workgroup for these tap counts and input type size
 size_t xy_ref_lds_bytes(int kx, int ky, int in_size);

@@ -9,8 +9,10 @@ images with the channel axis first, 2-D and 3-D images, factors 2..4 with
 ragged last blocks, every border, per-axis sigmas (an unblurred channel axis,
 a single blurred axis read straight from the source), wide kernels (taps
 beyond those carried in kernel arguments) and strided sources.  Each case
-also matches the two separate GPU calls bit for bit, and the march kernel
-(lanes marching along the last blurred axis) matches the generic one.
+also matches the two separate GPU calls bit for bit; the march kernel
+(lanes marching along the last blurred axis) matches the generic one, and
+the two-pass kernel with block means (the last two blurred axes, factor 2 or
+4) matches the march.
 """
 import sys
 
@@ -174,3 +176,42 @@ def test_generic_kernel_matches_the_march(case, monkeypatch):
     b = ndl_amd.gaussian_downsample(x, sigma, f, ca, border)
     assert same_bits(a, b)
     assert same_bits(a, oracle.downsample(oracle.gaussian_blur(x, sigma, border), f, ca))
+
+
+PAIR_CASES = [
+    # the last two blurred axes with the block means in one launch (factor 2
+    # or 4, every other axis the channel axis): colour images after their
+    # channel pass, 2-D images with one axis kept
+    ((30, 40, 3), 1.0, 2, 0, "clamp"),
+    ((31, 45, 3), 1.0, 2, 0, "reflect"),
+    ((70, 90, 3), 1.0, 4, 0, "wrap"),
+    ((64, 80), 1.6, 2, 1, "reflect"),
+    ((63, 81), 2.0, 4, 0, "clamp"),
+    ((100, 47), 0.9, 2, 1, "wrap"),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32, np.float64])
+@pytest.mark.parametrize("case", PAIR_CASES, ids=lambda c: f"{c[0]}-f{c[2]}-ca{c[3]}-{c[4]}")
+def test_two_pass_block_means_match(case, dtype, monkeypatch):
+    shape, sigma, f, ca, border = case
+    rng = np.random.default_rng(67)
+    x = data(dtype, shape, rng)
+    say("blur-downsample two-pass", np.dtype(dtype).name, shape, f, ca, border)
+    a = ndl_amd.gaussian_downsample(x, sigma, f, ca, border)
+    monkeypatch.setenv("NDL_XY_DOWN", "0")
+    b = ndl_amd.gaussian_downsample(x, sigma, f, ca, border)
+    assert same_bits(a, b)
+    assert same_bits(a, oracle.downsample(oracle.gaussian_blur(x, sigma, border), f, ca))
+
+
+def test_two_pass_block_means_strided_source():
+    rng = np.random.default_rng(68)
+    full = data(np.uint8, (90, 140), rng)
+    view = (slice(None, None, -1), slice(3, 133))
+    dense = np.ascontiguousarray(full[view])
+    want = oracle.downsample(oracle.gaussian_blur(dense, 1.2, "reflect"), 2, 1)
+    s = ndl_amd.DeviceArray.from_numpy(full).view(view)
+    d = ndl_amd.DeviceArray(want.shape, np.uint8)
+    ndl_amd.gaussian_downsample_dev(s, d, 1.2, 2, 1, "reflect", REF, NEAR)
+    assert same_bits(d.to_numpy(), want)
