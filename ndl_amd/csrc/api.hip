@@ -386,6 +386,51 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
     return e == hipSuccess ? NDL_OK : hip_fail(e, "two-pass reference launch");
 }
 
+// One pass along an axis of at most 16 elements on the short-axis kernel
+// (kernels_short.hip); -1 when it does not apply (a longer axis, more than
+// 64 nonzero taps or taps beyond the argument table, 2^32 lines or more, or
+// element types without an instance).
+int try_short_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& k, int accum,
+                   int rounding, hipStream_t s)
+{
+    if (!env_int("NDL_SHORT_AXIS", 1)) return -1;
+    const int64_t m = in->extent[axis];
+    if (m < 1 || m > kShortMax || k.count > kArgTaps) return -1;
+    const int64_t n = numel(in);
+    if (n == 0 || n / m > 0xffffffffLL) return -1;
+    ShortAxisArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = in->data;
+    a.out = out->data;
+    a.m = (int32_t)m;
+    a.in_as = in->stride[axis];
+    a.out_as = out->stride[axis];
+    a.rounding = rounding;
+    for (int t = 0; t < k.count; t++) {
+        if (k.taps[t] == 0.0) continue;
+        for (int j = 0; j < m; j++)
+            a.idx[j][a.nz] = (uint8_t)border_map(k.border, m, (int64_t)j + t - k.center);
+        a.wd[a.nz] = k.taps[t];
+        a.wf[a.nz++] = (float)k.taps[t];
+    }
+    int order[NDL_MAX_DIM], no = 0;
+    for (int d = 0; d < in->ndim; d++)
+        if (d != axis) order[no++] = d;
+    std::stable_sort(order, order + no, [&](int x, int y) { return std::llabs(in->stride[x]) < std::llabs(in->stride[y]); });
+    a.nd = no;
+    for (int i = 0; i < no; i++) {
+        const int d = order[i];
+        a.ext[i] = in->extent[d];
+        a.ist[i] = in->stride[d];
+        a.ost[i] = out->stride[d];
+        a.fdiv[i].init((uint32_t)in->extent[d]);
+    }
+    a.lines = (uint32_t)(n / m);
+    hipError_t e = launch_short_axis(a, in->dtype, out->dtype, accum, s);
+    if (e == hipErrorInvalidValue) return -1;
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "short-axis pass launch");
+}
+
 // per-axis chain through dense accumulator-typed intermediates; in
 // reference mode consecutive pairs of passes run as one launch where the
 // two-pass kernel applies
@@ -432,6 +477,7 @@ int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxe
             if (rc) return rc;
         }
         int rc = try_one_axis(cur, out, axes[i], k[i], accum, s, 0, nullptr, last ? rounding : NDL_ROUND_TRUNC);
+        if (rc == -1) rc = try_short_axis(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, s);
         if (rc == -1)
             rc = run_axis_pass(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, PASS_GATHER,
                                nullptr, 0, 0, s);

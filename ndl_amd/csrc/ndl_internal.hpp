@@ -46,6 +46,34 @@ struct AxisPassArgs {
 // accum: NDL_ACCUM_REF (FP64, separately rounded) or NDL_ACCUM_F32 (FP32 FMA)
 hipError_t launch_axis_pass(const AxisPassArgs& a, int in_dt, int out_dt, int accum, hipStream_t s);
 
+// ------------------------------------------------------- short-axis pass
+// One pass along an axis of at most 16 elements (the channel axis of a
+// colour image): a thread owns one line (every other index fixed) and writes
+// all m outputs (kernels_short.hip).  Output j is
+// sum over the nonzero taps n, ascending, of x[idx[j][n]] * w[n]: the border
+// fold is resolved on the host into element indices.
+constexpr int kShortMax = 16;
+struct ShortAxisArgs {
+    const void* in;
+    void* out;
+    int32_t m;                        // extent of the axis
+    int64_t in_as, out_as;            // strides along the axis
+    int32_t nd;                       // other dimensions, fastest (by input stride) first
+    int64_t ext[NDL_MAX_DIM];
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t lines;
+    int32_t rounding;
+    int32_t nz;                       // nonzero taps
+    double wd[kArgTaps];
+    float wf[kArgTaps];
+    uint8_t idx[kShortMax][kArgTaps]; // element read by output j at nonzero tap n
+};
+// REF: TIn -> FP64, FP64 -> TOut or FP64 -> FP64; F32: f32/f16/bf16 to the
+// same type.  hipErrorInvalidValue: no instance.
+hipError_t launch_short_axis(const ShortAxisArgs& a, int in_dt, int out_dt, int accum, hipStream_t s);
+
 // ------------------------------------------------------ dense N-D kernel
 struct NdArgs {
     const void* in;
