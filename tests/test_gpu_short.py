@@ -5,7 +5,8 @@ output).  Each case is bit-identical to the oracle and to the per-axis pass
 kernel (NDL_SHORT_AXIS=0): axis extents 1..16, every border with taps
 reaching up to the extent (clamp: beyond it), zero taps, off-centre kernels,
 the axis first, in the middle and last, strided views, the reference chain
-into and out of its FP64 intermediate, and FP32 accumulation.
+into and out of its FP64 intermediate, and FP32 accumulation; axes of at
+most 4 elements also against the run-time-tap form (NDL_SHORT4=0).
 """
 import sys
 
@@ -42,6 +43,12 @@ def check(x, axes, taps, borders, monkeypatch, accum=REF, centers=None, view=Non
     a = run(x, axes, taps, borders, accum, rounding, monkeypatch, True, centers, view)
     b = run(x, axes, taps, borders, accum, rounding, monkeypatch, False, centers, view)
     np.testing.assert_array_equal(a.view(np.uint8), b.view(np.uint8))
+    # axes of at most 4 elements: the compile-time-tap form against the
+    # run-time one
+    monkeypatch.setenv("NDL_SHORT4", "0")
+    c = run(x, axes, taps, borders, accum, rounding, monkeypatch, True, centers, view)
+    monkeypatch.delenv("NDL_SHORT4")
+    np.testing.assert_array_equal(a.view(np.uint8), c.view(np.uint8))
     if accum == REF:
         sub = np.ascontiguousarray(x[view]) if view is not None else x
         want = oracle.sepconv(sub, axes, taps, borders, rounding, centers=centers)
