@@ -97,20 +97,26 @@ __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
     const TIn* in = (const TIn*)a.in + ib;
 
     // 1. input window: every load of the wave is issued before the first
-    //    LDS store (rows are compile-time register slots)
+    //    LDS store (rows are compile-time register slots).  Lanes past the
+    //    window's right end repeat its last element (same address, same
+    //    value), so no load or store is predicated per lane; rows fold only
+    //    in tiles that reach past the first or last row.
     {
         constexpr int RW = (HA + 3) / 4; // window rows per wave, at most
         const int64_t xo0 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + lane) * a.isx;
-        const bool two = lane + 64 < WA;
-        const int64_t xo1 = two ? (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + lane + 64) * a.isx : 0;
+        const int c1 = min(lane + 64, WA - 1);
+        const int64_t xo1 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + c1) * a.isx;
+        const int ybase = y0 - a.cy;
+        const bool yin = ybase >= 0 && ybase + HA <= a.Y;
         TIn e0[RW], e1[RW];
 #pragma unroll
         for (int j = 0; j < RW; j++) {
             const int r = wv + 4 * j;
             if (r < HA) {
-                const TIn* row = in + (int64_t)safe_map32(a.by, a.Y, y0 - a.cy + r) * a.isy;
+                const int yy = yin ? ybase + r : safe_map32(a.by, a.Y, ybase + r);
+                const TIn* row = in + (int64_t)yy * a.isy;
                 e0[j] = row[xo0];
-                if (two) e1[j] = row[xo1];
+                e1[j] = row[xo1];
             }
         }
 #pragma unroll
@@ -118,7 +124,7 @@ __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
             const int r = wv + 4 * j;
             if (r < HA) {
                 win[r * WA + lane] = e0[j];
-                if (two) win[r * WA + lane + 64] = e1[j];
+                win[r * WA + c1] = e1[j];
             }
         }
     }
