@@ -382,6 +382,56 @@ def suite(reps=10, configs=(1, 2, 3, 4, 5, 6)):
         emit({"config": "2a reference arithmetic: 256^3 f32 Gaussian 11/11/19 reflect, FP64 chain, cold",
               "us": round(ms * 1e3, 2), "gvox_s": round(x.size / ms / 1e6, 2),
               "bit_exact": bool(np.array_equal(d.to_numpy(), oracle.sepconv(x, axes, taps, borders)))})
+        # the drop-in gaussian_blur() on a uint8 colour image (every axis,
+        # the channel axis included, rounded to nearest), and one pyramid
+        # step (blur then downsample by 2) of the 256^3 volume, both in the
+        # reference arithmetic; the pyramid step against the two calls
+        rng = np.random.default_rng(7)
+        xc = rng.integers(0, 256, size=(2048, 2048, 3)).astype(np.uint8)
+        sc = ndl_amd.DeviceArray.from_numpy(xc)
+        dc = ndl_amd.DeviceArray(xc.shape, np.uint8)
+        g1 = workloads.gaussian_taps(1.0)
+        ks = abi.Kernels([g1] * 3, ["clamp"] * 3)
+        ax3 = abi.Kernels.axes([0, 1, 2])
+        tsc, tdc = sc.tensor(), dc.tensor()
+
+        def fc():
+            ndl_amd.check(lib.ndl_sepconv_forward(C.byref(tsc), C.byref(tdc), 3, ax3, ks.arr, abi.ACCUM_REF,
+                                                  abi.ROUND_NEAREST, stream))
+        fc()
+        ms = _time_launches(hip, stream, fc, reps, flush)
+        emit({"config": "gaussian_blur uint8 colour {3,2048,2048}, sigma 1, clamp, reference arithmetic, cold",
+              "us": round(ms * 1e3, 2), "gvox_s": round(xc.size / ms / 1e6, 2),
+              "bit_exact": bool(np.array_equal(dc.to_numpy(), oracle.gaussian_blur(xc, 1.0, "clamp")))})
+        del sc, dc
+        sig = np.full(3, 1.5)
+        oshape = (128, 128, 256)
+        dd = ndl_amd.DeviceArray(oshape, np.float32)
+        mid = ndl_amd.DeviceArray(x.shape, np.float32)
+        g15 = workloads.gaussian_taps(1.5)
+        kb = abi.Kernels([g15] * 3, ["reflect"] * 3)
+        ts_, tdd, tmid = s.tensor(), dd.tensor(), mid.tensor()
+
+        def fpyr():
+            ndl_amd.check(lib.ndl_gaussian_downsample(C.byref(ts_), C.byref(tdd), sig.ctypes.data_as(C.POINTER(C.c_double)),
+                                                      abi.BORDERS["reflect"], 2, 0, abi.ACCUM_REF, abi.ROUND_TRUNC,
+                                                      stream))
+
+        def ftwo():
+            ndl_amd.check(lib.ndl_sepconv_forward(C.byref(ts_), C.byref(tmid), 3, ax3, kb.arr, abi.ACCUM_REF,
+                                                  abi.ROUND_TRUNC, stream))
+            ndl_amd.check(lib.ndl_downsample(C.byref(tmid), C.byref(tdd), 2, 0, stream))
+        fpyr()
+        ms = _time_launches(hip, stream, fpyr, reps, flush)
+        got = dd.to_numpy()
+        ftwo()
+        ms2 = _time_launches(hip, stream, ftwo, reps, flush)
+        emit({"config": "pyramid step: 256^3 f32 gaussian_blur sigma 1.5 reflect then downsample by 2 (x kept), "
+                        "reference arithmetic, cold",
+              "us": round(ms * 1e3, 2), "us_blur_then_downsample": round(ms2 * 1e3, 2),
+              "gvox_s": round(x.size / ms / 1e6, 2),
+              "bit_exact_vs_two_calls": bool(np.array_equal(got, dd.to_numpy()))})
+        del dd, mid
     if 3 in configs:
         x, axes, taps, borders = workloads.config3()
         s = ndl_amd.DeviceArray.from_numpy(x)
