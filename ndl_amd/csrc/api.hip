@@ -388,8 +388,7 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
 
 // One pass along an axis of at most 16 elements on the short-axis kernel
 // (kernels_short.hip); -1 when it does not apply (a longer axis, more than
-// 64 nonzero taps or taps beyond the argument table, 2^32 lines or more, or
-// element types without an instance).
+// 64 taps, 2^32 lines or more, or element types without an instance).
 int try_short_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& k, int accum,
                    int rounding, hipStream_t s)
 {
@@ -406,12 +405,12 @@ int try_short_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const 
     a.in_as = in->stride[axis];
     a.out_as = out->stride[axis];
     a.rounding = rounding;
+    a.k = k.count;
+    a.center = k.center;
+    a.border = k.border;
     for (int t = 0; t < k.count; t++) {
-        if (k.taps[t] == 0.0) continue;
-        for (int j = 0; j < m; j++)
-            a.idx[j][a.nz] = (uint8_t)border_map(k.border, m, (int64_t)j + t - k.center);
-        a.wd[a.nz] = k.taps[t];
-        a.wf[a.nz++] = (float)k.taps[t];
+        a.wd[t] = k.taps[t];
+        a.wf[t] = (float)k.taps[t];
     }
     int order[NDL_MAX_DIM], no = 0;
     for (int d = 0; d < in->ndim; d++)
@@ -1566,8 +1565,9 @@ NDL_API int ndl_sepconv_forward(const ndl_tensor* src, const ndl_tensor* dst, in
 // Validates an N-D kernel call and fills NdArgs with the included taps
 // (detail::kernelIncludedTaps: kernel coordinates dimension 0 fastest,
 // nonzero entries only, offsets relative to the centre) uploaded into tb.
+// checks_only: validate and set a.total only (no tap upload)
 int prepare_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel, const int64_t* kextent,
-               int32_t border, Scratch& tb, NdArgs& a, hipStream_t s)
+               int32_t border, Scratch& tb, NdArgs& a, hipStream_t s, bool checks_only = false)
 {
     int rc;
     if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
@@ -1588,6 +1588,10 @@ int prepare_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kerne
     int64_t n = numel(src);
     if (n == 0) return NDL_OK;
     if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    if (checks_only) {
+        a.total = (uint32_t)n;
+        return NDL_OK;
+    }
     std::vector<int32_t> offs;
     std::vector<double> vals;
     for (int64_t lin = 0; lin < kn; lin++) {
@@ -1624,12 +1628,104 @@ int prepare_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kerne
     return NDL_OK;
 }
 
+// An integer image convolved with an integer rank-1 kernel (a box from
+// make_box_kernel, outer products of binomials, ...): the reference's single
+// FP64 sum over the kernel (convolution.h:52-71) is exact, since every
+// product and partial sum is an integer below 2^53, so it equals the exact
+// result of one pass per axis; the separable path (the packed-byte kernel for
+// uint8, the reference chain otherwise) then gives the identical value and
+// cast.  -1 when the kernel is not such a product or the bound fails.
+int try_separable_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel, const int64_t* kextent,
+                     int32_t border, hipStream_t s)
+{
+    if (!env_int("NDL_ND_SEPARABLE", 1)) return -1;
+    const int it = src->dtype;
+    double vmax;
+    switch (it) {
+    case NDL_U8: vmax = 255.0; break;
+    case NDL_I8: vmax = 128.0; break;
+    case NDL_U16: vmax = 65535.0; break;
+    case NDL_I16: vmax = 32768.0; break;
+    case NDL_I32: vmax = 2147483648.0; break;
+    default: return -1;
+    }
+    const int nd = src->ndim;
+    int64_t kn = 1;
+    for (int d = 0; d < nd; d++) kn *= kextent[d];
+    if (kn < 2 || kn > (1 << 22)) return -1;
+    int64_t p = -1;
+    for (int64_t i = 0; i < kn; i++) {
+        const double w = kernel[i];
+        if (w != std::floor(w) || !(std::fabs(w) <= 1048576.0)) return -1;
+        if (p < 0 && w != 0.0) p = i;
+    }
+    if (p < 0) return -1;
+    // the kernel's lines through its first nonzero entry; rank 1 with
+    // integer factors when k[p] divides every other line
+    int64_t pc[NDL_MAX_DIM], kst[NDL_MAX_DIM];
+    int64_t r = p, st = 1;
+    for (int d = 0; d < nd; d++) {
+        pc[d] = r % kextent[d];
+        r /= kextent[d];
+        kst[d] = st;
+        st *= kextent[d];
+    }
+    const double kp = kernel[p];
+    std::vector<std::vector<double>> fac(nd);
+    for (int d = 0; d < nd; d++) {
+        fac[d].resize(kextent[d]);
+        for (int64_t i = 0; i < kextent[d]; i++) {
+            const double v = kernel[p + (i - pc[d]) * kst[d]];
+            const double f = d == 0 ? v : v / kp;
+            if (f != std::floor(f)) return -1;
+            fac[d][i] = f;
+        }
+    }
+    double bound = vmax;
+    for (int d = 0; d < nd; d++) {
+        double sa = 0;
+        for (double f : fac[d]) sa += std::fabs(f);
+        bound *= sa;
+    }
+    if (!(bound < 9007199254740992.0)) return -1;
+    for (int64_t i = 0; i < kn; i++) {
+        double prod = 1;
+        int64_t q = i;
+        for (int d = 0; d < nd; d++) {
+            prod *= fac[d][q % kextent[d]];
+            q /= kextent[d];
+        }
+        if (prod != kernel[i]) return -1;
+    }
+    int32_t axes[NDL_MAX_DIM];
+    ndl_axis_kernel k[NDL_MAX_DIM];
+    int na = 0;
+    for (int d = 0; d < nd; d++) {
+        if (kextent[d] == 1) {
+            if (fac[d][0] != 1.0) return -1; // a scale on a unit axis: keep the N-D sum's single product
+            continue;
+        }
+        if (kextent[d] > NDL_MAX_TAPS) return -1;
+        axes[na] = d;
+        k[na++] = ndl_axis_kernel{fac[d].data(), (int32_t)kextent[d], (int32_t)(kextent[d] / 2), border};
+    }
+    if (na == 0) return -1;
+    return forward_impl(src, dst, na, axes, k, NDL_ACCUM_REF, NDL_ROUND_TRUNC, s);
+}
+
 NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel,
                             const int64_t* kextent, int32_t border, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
     Scratch tb;
     NdArgs a;
+    {
+        // argument checks first (the same errors on every route)
+        int rc = prepare_nd(src, dst, kernel, kextent, border, tb, a, s, true);
+        if (rc || a.total == 0) return rc;
+        rc = try_separable_nd(src, dst, kernel, kextent, border, s);
+        if (rc != -1) return rc;
+    }
     int rc = prepare_nd(src, dst, kernel, kextent, border, tb, a, s);
     if (rc || a.total == 0) return rc;
     hipError_t e = launch_convolve_nd(a, src->dtype, dst->dtype, s);
