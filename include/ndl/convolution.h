@@ -387,7 +387,7 @@ auto gaussian_downsample(const ImageT& src, double sigma, int factor, int channe
 /// Spatial gradient (processing/convolution.h:359-388): dst.slice(0, axis) is
 /// the central difference {-0.5, 0, +0.5} along `axis` (the zero tap is
 /// skipped, as in convolve()), for every axis; dst has a leading component
-/// axis of size DIM.
+/// axis of size DIM.  All components in one launch.
 template <class SrcImageT, class DstImageT>
 void gradient(const SrcImageT& src, DstImageT& dst, BorderMode border = BorderMode::Reflect)
 {
@@ -398,12 +398,10 @@ void gradient(const SrcImageT& src, DstImageT& dst, BorderMode border = BorderMo
     static_assert(DstDIM == DIM + 1, "ndl::gradient() needs a destination with a leading component axis");
     assert(dst.extent()[0] == DIM);
     for (int d = 0; d < DIM; d++) assert(dst.extent()[d + 1] == src.extent()[d]);
-    for (int axis = 0; axis < DIM; axis++) {
-        auto component = dst.slice(0, axis);
-        detail::AxisPlan<DIM> plan;
-        plan.add(axis, {-0.5, 0.0, 0.5}, 1, border);
-        detail::run_sepconv(src, component, plan, Accumulation::Reference, NDL_ROUND_TRUNC);
-    }
+    detail::DeviceSource<SrcImageT> s(src);
+    detail::DeviceDest<DstImageT> o(dst);
+    detail::check(ndl_gradient(&s.t, &o.t, detail::border_code(border), nullptr));
+    o.finish();
 }
 
 /// Input gradient of L = <w, convolve_separable(x)>: grad_in = A^T grad_out,

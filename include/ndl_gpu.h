@@ -124,7 +124,10 @@ int ndl_sepconv_forward(const ndl_tensor* src, const ndl_tensor* dst, int32_t na
  * convolve(src, dst, kernel, border) for an arbitrary (non-separable) N-D
  * kernel, convolution.h:52-71: taps in kernel coordinate order (dimension 0
  * fastest), zero taps skipped, FP64 accumulation, static_cast output.
- * kernel: host pointer to prod(kextent) doubles, dense, dimension 0 fastest. */
+ * kernel: host pointer to prod(kextent) doubles, dense, dimension 0 fastest.
+ * Integer images with an integer rank-1 kernel (a box, outer products of
+ * binomials), whose FP64 sums are exact, run as one pass per axis: the same
+ * result. */
 int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const double* kernel,
                     const int64_t* kextent, int32_t border, void* stream);
 
@@ -167,6 +170,14 @@ int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels, int32_t
  * stored as dogs' element type (exact for float64 levels, as the reference's
  * OwnedImage<double> layers).  Equal extents, no overlap with the levels. */
 int ndl_dog(const ndl_tensor* levels, const ndl_tensor* dogs, int32_t nlevels, void* stream);
+
+/* ---- gradient --------------------------------------------------------------
+ * gradient(src, dst, border), convolution.h:359-388: dst has one more
+ * dimension than src, a leading component axis of src->ndim entries, and
+ * dst[a, p] = convolve() of src with {-0.5, 0, 0.5} along axis a at p (FP64,
+ * the zero tap skipped, truncating cast): bit-identical to the reference's
+ * one convolve() per axis, all components in one launch. */
+int ndl_gradient(const ndl_tensor* src, const ndl_tensor* dst, int32_t border, void* stream);
 
 /* ---- downsample ------------------------------------------------------------
  * downsample(src, factor, channelAxis), convolution.h:241-305: every axis but

@@ -40,6 +40,7 @@ SIGNATURES = {
     "ndl_convolve_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, _VP]),
     "ndl_morph_separable": (C.c_int, [_T, _T, C.c_int32, _I32P, _K, C.c_int32, _VP]),
     "ndl_morph_nd": (C.c_int, [_T, _T, _DP, _I64P, C.c_int32, C.c_int32, _VP]),
+    "ndl_gradient": (C.c_int, [_T, _T, C.c_int32, _VP]),
     "ndl_downsample": (C.c_int, [_T, _T, C.c_int32, C.c_int32, _VP]),
     "ndl_gaussian_downsample": (C.c_int, [_T, _T, _DP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           _VP]),
@@ -263,6 +264,14 @@ def dog_dev(levels, dogs, stream=None):
     la = (abi.Tensor * len(levels))(*[lv.tensor() for lv in levels])
     da = (abi.Tensor * max(1, len(dogs)))(*[d.tensor() for d in dogs])
     return check(lib().ndl_dog(la, da, len(levels), stream))
+
+
+def gradient_dev(src, dst, border="reflect", stream=None):
+    """ndl::gradient for DeviceArrays: dst has a leading component axis (the
+    last numpy axis, of size src's ndim), component a = d/dx_a (ndl axis a)."""
+    b = abi.BORDERS[border] if isinstance(border, str) else int(border)
+    ts, td = src.tensor(), dst.tensor()
+    return check(lib().ndl_gradient(C.byref(ts), C.byref(td), b, stream))
 
 
 def downsample_dev(src, dst, factor, channel_axis=0, stream=None):

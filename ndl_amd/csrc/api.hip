@@ -1733,6 +1733,44 @@ NDL_API int ndl_convolve_nd(const ndl_tensor* src, const ndl_tensor* dst, const 
     return NDL_OK;
 }
 
+// ---------------------------------------------------------------- gradient
+NDL_API int ndl_gradient(const ndl_tensor* src, const ndl_tensor* dst, int32_t border, void* stream)
+{
+    int rc;
+    if ((rc = check_tensor(src, "src")) || (rc = check_tensor(dst, "dst"))) return rc;
+    const int nd = src->ndim;
+    if (nd + 1 > NDL_MAX_DIM || dst->ndim != nd + 1)
+        return fail(NDL_ERR_SHAPE, "gradient needs dst ndim = src ndim + 1 <= %d", NDL_MAX_DIM);
+    if (border < NDL_CLAMP || border > NDL_REFLECT) return fail(NDL_ERR_ARG, "unknown border %d", border);
+    if (dst->extent[0] != nd)
+        return fail(NDL_ERR_SHAPE, "dst component axis has %lld entries, expected %d", (long long)dst->extent[0], nd);
+    for (int d = 0; d < nd; d++)
+        if (dst->extent[d + 1] != src->extent[d])
+            return fail(NDL_ERR_SHAPE, "dst extent %lld on dimension %d, expected %lld", (long long)dst->extent[d + 1],
+                        d + 1, (long long)src->extent[d]);
+    if ((rc = check_no_overlap(src, dst))) return rc;
+    const int64_t n = numel(src);
+    if (n == 0) return NDL_OK;
+    if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
+    GradArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = src->data;
+    a.out = dst->data;
+    a.nd = nd;
+    for (int d = 0; d < nd; d++) {
+        a.ext[d] = src->extent[d];
+        a.ist[d] = src->stride[d];
+        a.ost[d] = dst->stride[d + 1];
+        a.fdiv[d].init((uint32_t)src->extent[d]);
+    }
+    a.cst = dst->stride[0];
+    a.total = (uint32_t)n;
+    a.border = border;
+    hipError_t e = launch_gradient(a, src->dtype, dst->dtype, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "gradient launch");
+    return NDL_OK;
+}
+
 // ---------------------------------------------------------------- downsample
 NDL_API int ndl_downsample(const ndl_tensor* src, const ndl_tensor* dst, int32_t factor, int32_t channel_axis,
                            void* stream)

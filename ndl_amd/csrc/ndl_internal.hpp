@@ -48,10 +48,8 @@ hipError_t launch_axis_pass(const AxisPassArgs& a, int in_dt, int out_dt, int ac
 
 // ------------------------------------------------------- short-axis pass
 // One pass along an axis of at most 16 elements (the channel axis of a
-// colour image): a thread owns one line (every other index fixed) and writes
-// all m outputs (kernels_short.hip).  Output j is
-// sum over the nonzero taps n, ascending, of x[idx[j][n]] * w[n]: the border
-// fold is resolved on the host into element indices.
+// colour image): a thread owns one line (every other index fixed), holds its
+// m elements in registers and writes all m outputs (kernels_short.hip).
 constexpr int kShortMax = 16;
 struct ShortAxisArgs {
     const void* in;
@@ -65,10 +63,9 @@ struct ShortAxisArgs {
     FastDiv fdiv[NDL_MAX_DIM];
     uint32_t lines;
     int32_t rounding;
-    int32_t nz;                       // nonzero taps
+    int32_t k, center, border;        // taps, centre, border mode
     double wd[kArgTaps];
     float wf[kArgTaps];
-    uint8_t idx[kShortMax][kArgTaps]; // element read by output j at nonzero tap n
 };
 // REF: TIn -> FP64, FP64 -> TOut or FP64 -> FP64; F32: f32/f16/bf16 to the
 // same type.  hipErrorInvalidValue: no instance.
@@ -205,6 +202,25 @@ hipError_t launch_blur_downsample(const BlurDownArgs& a, int in_dt, int out_dt, 
 // in 3..25 with no zero tap and P <= 64: lanes march along the axis with a
 // register ring (kernels_line.hip).  hipErrorInvalidValue: no instance.
 hipError_t launch_blur_down_march(const BlurDownArgs& a, int accum, hipStream_t s);
+
+// ---------------------------------------------------------------- gradient
+// gradient(), convolution.h:359-388: every component in one launch.  One
+// thread per source element; component a is the reference's convolve() with
+// {-0.5, 0, 0.5} along a (the zero tap skipped): total = 0 + x[B(i-1)] *
+// -0.5, then total + x[B(i+1)] * 0.5, in FP64, cast with truncation.
+struct GradArgs {
+    const void* in;
+    void* out;
+    int32_t nd;                 // source dimensions
+    int64_t ext[NDL_MAX_DIM];
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];   // dst strides of the spatial dimensions (dst dims 1..nd)
+    int64_t cst;                // dst stride of the component axis (dst dim 0)
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t total;
+    int32_t border;
+};
+hipError_t launch_gradient(const GradArgs& a, int in_dt, int out_dt, hipStream_t s);
 
 // ------------------------------------------- two reference passes at once
 // Two consecutive passes of the NDL_ACCUM_REF chain (kernels_xyref.hip):
