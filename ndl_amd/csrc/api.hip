@@ -796,7 +796,11 @@ bool one_axis_types(const ndl_tensor* in, const ndl_tensor* out, const ndl_axis_
     } else if (accum == NDL_ACCUM_F32) {
         // float, bf16 or f16 storage, the same type in and out
         if (it != ot || (it != NDL_F32 && it != NDL_BF16 && it != NDL_F16)) return false;
-    } else if (accum != NDL_ACCUM_REF || (it != NDL_F32 && it != NDL_F64) ||
+    } else if (accum != NDL_ACCUM_REF) {
+        return false;
+    } else if (it == NDL_U8) {
+        if (ot != NDL_U8) return false; // a single pass over a uint8 image
+    } else if ((it != NDL_F32 && it != NDL_F64) ||
                !(ot == NDL_F32 || ot == NDL_F64 ||
                  (it == NDL_F64 && (ot == NDL_U8 || ot == NDL_U16 || ot == NDL_I16)))) {
         return false;
@@ -1807,6 +1811,30 @@ NDL_API int ndl_downsample(const ndl_tensor* src, const ndl_tensor* dst, int32_t
         a.fdiv[d].init((uint32_t)dst->extent[d]);
     }
     a.total = (uint32_t)n;
+    // blocks of at most 16 positions: their offsets and steps, in block order
+    int64_t P = 1;
+    for (int d = 0; d < a.nd; d++) {
+        if (d == channel_axis || src->extent[d] == 1) continue;
+        if (a.nq == 4 || (P *= factor) > kDownPos) {
+            P = 0;
+            break;
+        }
+        a.qdim[a.nq++] = d;
+    }
+    if (P > 0 && env_int("NDL_DOWN_FAST", 1)) {
+        a.P = (int32_t)P;
+        for (int p = 0; p < a.P; p++) {
+            int r = p;
+            for (int k = 0; k < a.nq; k++) {
+                a.pdig[p][k] = (uint8_t)(r % factor);
+                r /= factor;
+                a.poff[p] += (int64_t)a.pdig[p][k] * src->stride[a.qdim[k]];
+            }
+        }
+        hipError_t e = launch_downsample_fast(a, src->dtype, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "downsample launch");
+        return NDL_OK;
+    }
     hipError_t e = launch_downsample(a, src->dtype, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "downsample launch");
     return NDL_OK;

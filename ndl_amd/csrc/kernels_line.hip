@@ -307,6 +307,8 @@ template <class F> hipError_t with_types(const LineKey& key, F&& f)
     if (key.in_dt == NDL_F32 && key.out_dt == NDL_F64) return f(float{}, double{}, double{}, IC<0>{});
     if (key.in_dt == NDL_F64 && key.out_dt == NDL_F64) return f(double{}, double{}, double{}, IC<0>{});
     if (key.in_dt == NDL_F64 && key.out_dt == NDL_F32) return f(double{}, float{}, double{}, IC<0>{});
+    // a single pass over a uint8 image (convolve() with a one-axis kernel)
+    if (key.in_dt == NDL_U8 && key.out_dt == NDL_U8) return f(uint8_t{}, uint8_t{}, double{}, IC<0>{});
     // the last pass of a chain into an integral image (rounded per a.rounding)
     if (key.in_dt == NDL_F64 && key.out_dt == NDL_U8) return f(double{}, uint8_t{}, double{}, IC<0>{});
     if (key.in_dt == NDL_F64 && key.out_dt == NDL_U16) return f(double{}, uint16_t{}, double{}, IC<0>{});

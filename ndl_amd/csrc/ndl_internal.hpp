@@ -161,8 +161,19 @@ struct DownArgs {
     int64_t ost[NDL_MAX_DIM];
     FastDiv fdiv[NDL_MAX_DIM]; // by the output extents, dimension 0 first
     uint32_t total;
+    // blocks of at most kDownPos positions (kernels_downsample.hip): the
+    // decimated dimensions with extent > 1 (dimension 0 first), and per
+    // position of a full block, in block order, its offset and its step along
+    // each of them (positions past a ragged edge are skipped)
+    int32_t P, nq;
+    int32_t qdim[4];
+    int64_t poff[16];
+    uint8_t pdig[16][4];
 };
+constexpr int kDownPos = 16;
 hipError_t launch_downsample(const DownArgs& a, int dt, hipStream_t s);
+// hipErrorInvalidValue when a.P is 0 or above kDownPos
+hipError_t launch_downsample_fast(const DownArgs& a, int dt, hipStream_t s);
 
 // The last axis pass of a blur fused with the block mean: per output, each
 // source position of its block in block order (dimension 0 fastest) gets the

@@ -157,6 +157,24 @@ def test_downsample_vs_oracle(dtype):
         assert same_bits(got, want), (np.dtype(dtype).name, shape, f, ca)
 
 
+@pytest.mark.parametrize("fast", ["1", "0"])
+def test_downsample_unrolled_and_generic_kernels(fast, monkeypatch):
+    # blocks of at most 16 positions take the unrolled kernel (NDL_DOWN_FAST=1),
+    # larger ones the generic walk; both match the oracle bit for bit
+    monkeypatch.setenv("NDL_DOWN_FAST", fast)
+    rng = np.random.default_rng(58)
+    for dtype in (np.uint8, np.int16, np.float32, np.float64):
+        for shape, f, ca in [((31, 45, 3), 2, 0), ((9, 10, 11, 12), 2, 3), ((37, 41), 4, 1), ((5, 50, 51), 3, 2),
+                             ((13, 1, 17), 2, 0), ((7, 8, 9, 10), 3, 0)]:
+            if np.issubdtype(dtype, np.floating):
+                x = (rng.standard_normal(shape) * 100).astype(dtype)
+            else:
+                info = np.iinfo(dtype)
+                x = rng.integers(info.min, info.max, size=shape, endpoint=True).astype(dtype)
+            np.testing.assert_array_equal(ndl_amd.downsample(x, f, ca).view(np.uint8),
+                                          oracle.downsample(x, f, ca).view(np.uint8))
+
+
 def test_downsample_reference_constant_colour():
     # image_convolution_tests.cpp:70-91
     x = np.full((30, 40, 3), 77, dtype=np.uint8)

@@ -544,12 +544,13 @@ def test_row_pass_x_axis(case, border, monkeypatch):
     assert rel_err(tile, want) <= 1e-6
 
 
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.uint8])
 @pytest.mark.parametrize("case", [("line", i) for i in range(len(LINE_CASES))] +
                          [("row", i) for i in range(len(ROW_CASES))], ids=lambda c: f"{c[0]}{c[1]}")
 def test_one_axis_kernels_reference_bit_exact(case, dtype, monkeypatch):
     # the reference's FP64 arithmetic (NDL_ACCUM_REF) on the line march and
-    # the row kernel: bit-identical to the oracle for float and double data
+    # the row kernel: bit-identical to the oracle for float, double and uint8
+    # data (uint8: float taps, the truncating cast of convolve())
     kind, i = case
     if kind == "line":
         shape, axis, k, view, env = LINE_CASES[i]
@@ -560,7 +561,7 @@ def test_one_axis_kernels_reference_bit_exact(case, dtype, monkeypatch):
     border = ["clamp", "wrap", "reflect"][i % 3]
     say("one axis REF", kind, shape, axis, k, border, dtype.__name__, env)
     rng = np.random.default_rng(300 + i)
-    base = rng.random(shape).astype(dtype)
+    base = (rng.integers(0, 256, size=shape) if dtype == np.uint8 else rng.random(shape)).astype(dtype)
     v = base[view] if view is not None else base
     taps = [rng.random(k) + 0.05]
     centers = None if centre is None else [centre]
