@@ -837,7 +837,7 @@ bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_ax
 {
     if (!env_int("NDL_LINE_MARCH", 1) || !one_axis_types(in, out, ak, accum, key, op)) return false;
     const int nd = in->ndim, xd = unit_dim(in, out);
-    if (xd < 0 || xd == zd || in->extent[xd] < 64 || in->extent[zd] > 0x7fffffff) return false;
+    if (xd < 0 || xd == zd || in->extent[zd] > 0x7fffffff) return false;
     // the widest lanes (16 bytes at most, NDL_LINE_V caps them) whose
     // addresses are aligned and that have an instance
     int v = std::min(16 / (int)dtype_size(in->dtype), env_int("NDL_LINE_V", 4));
@@ -891,6 +891,18 @@ bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_ax
         lanes *= in->extent[e];
         la.nb++;
         if (lanes > 0xfffffe00LL) return false;
+    }
+    // a short unit-stride dimension (the channels of a {c, x, y} colour
+    // image) still gives coalesced lanes when the first lane dimension after
+    // it continues the run (lane = c + 3 x for the y pass of a colour image;
+    // not the x pass, whose lanes would step across rows); a small problem
+    // stays on the per-axis pass
+    if (in->extent[xd] < 64) {
+        if (lanes * la.Z < 16384 || la.nb == 0) return false;
+        int e0 = -1;
+        for (int e = 0; e < nd && e0 < 0; e++)
+            if (e != xd && e != zd && in->extent[e] > 1) e0 = e;
+        if (in->stride[e0] != in->extent[xd] || out->stride[e0] != in->extent[xd]) return false;
     }
     la.lanes = (uint32_t)lanes;
     // z chunks: fill the resident workgroup slots in whole rounds; a chunk

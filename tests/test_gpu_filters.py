@@ -257,6 +257,9 @@ MORPH_ONE_AXIS = [
     (np.float32, (300, 260), (5, 9)),
     (np.uint8, (3, 300, 700), (1, 7, 7)),
     (np.uint8, (40, 33, 96), (3, 11, 5)),
+    # interleaved colour {3, x, y}: lanes over channel and pixel
+    (np.uint8, (120, 200, 3), (5, 5, 1)),
+    (np.float32, (90, 150, 3), (3, 7, 1)),
 ]
 
 
