@@ -1833,7 +1833,7 @@ NDL_API int ndl_downsample(const ndl_tensor* src, const ndl_tensor* dst, int32_t
         }
         a.qdim[a.nq++] = d;
     }
-    if (P > 0 && env_int("NDL_DOWN_FAST", 1)) {
+    if (P > 0 && env_int("NDL_DOWN_FAST", 1) && (P == 2 || P == 3 || P == 4 || P == 8 || P == 9 || P == 16)) {
         a.P = (int32_t)P;
         for (int p = 0; p < a.P; p++) {
             int r = p;

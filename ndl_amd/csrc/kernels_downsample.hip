@@ -3,7 +3,7 @@
 // up to two).
 //
 // One output per thread, as the generic kernel, but the block's positions
-// are unrolled: their offsets (relative to the block's first element) and
+// (2, 3, 4, 8, 9 or 16, a template parameter) are unrolled: their offsets (relative to the block's first element) and
 // their steps along each decimated axis come from the host, so every load of
 // the block is issued before the first is summed.  A position past a ragged
 // far edge (its step reaches the extent) is skipped, as the reference's
@@ -15,7 +15,7 @@
 namespace ndl {
 namespace {
 
-template <class T> __global__ __launch_bounds__(256) void downsample_fast_kernel(const DownArgs a)
+template <class T, int P> __global__ __launch_bounds__(256) void downsample_fast_kernel(const DownArgs a)
 {
     const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
     if (idx >= a.total) return;
@@ -35,19 +35,26 @@ template <class T> __global__ __launch_bounds__(256) void downsample_fast_kernel
     }
     const int64_t count = (int64_t)lim[0] * lim[1] * lim[2] * lim[3];
     const T* in = (const T*)a.in + ioff;
-    double v[kDownPos];
-    bool ok[kDownPos];
+    // a full block (the common case) needs no per-position check
+    bool full = true;
 #pragma unroll
-    for (int p = 0; p < kDownPos; p++) {
-        ok[p] = p < a.P;
+    for (int k = 0; k < 4; k++)
+        if (k < a.nq && lim[k] < a.factor) full = false;
+    double v[P];
+    bool ok[P];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (k < a.nq && a.pdig[p][k] >= lim[k]) ok[p] = false;
+    for (int p = 0; p < P; p++) {
+        ok[p] = true;
+        if (!full) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < a.nq && a.pdig[p][k] >= lim[k]) ok[p] = false;
+        }
         v[p] = ok[p] ? load_f64(in + a.poff[p]) : 0.0;
     }
     double sum = 0.0;
 #pragma unroll
-    for (int p = 0; p < kDownPos; p++)
+    for (int p = 0; p < P; p++)
         if (ok[p]) sum = sum + v[p];
     store_from_f64((T*)a.out + ooff, sum / (double)count, NDL_ROUND_NEAREST);
 }
@@ -60,7 +67,16 @@ hipError_t launch_downsample_fast(const DownArgs& a, int dt, hipStream_t s)
     if (a.total == 0) return hipSuccess;
     const uint32_t blocks = (a.total + 255u) / 256u;
     auto go = [&](auto t) {
-        hipLaunchKernelGGL((downsample_fast_kernel<decltype(t)>), dim3(blocks), dim3(256), 0, s, a);
+        using T = decltype(t);
+        switch (a.P) {
+        case 2: hipLaunchKernelGGL((downsample_fast_kernel<T, 2>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((downsample_fast_kernel<T, 3>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((downsample_fast_kernel<T, 4>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((downsample_fast_kernel<T, 8>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 9: hipLaunchKernelGGL((downsample_fast_kernel<T, 9>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 16: hipLaunchKernelGGL((downsample_fast_kernel<T, 16>), dim3(blocks), dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     };
     switch (dt) {
