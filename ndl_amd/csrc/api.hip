@@ -2141,19 +2141,40 @@ NDL_API int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels,
         return NDL_OK;
     }
     if (n > 0xffffffffLL) return fail(NDL_ERR_SHAPE, "more than 2^32-1 elements in one call");
-    // exact mode: the axis-0 pass of every level from one read of the source
-    // (gaussian_blur's first FP64 buffer, convolution.h:151-169), then each
-    // level's remaining axes through the FP64 chain
+    // exact mode.  Levels whose first two passes fit the two-pass kernel run
+    // the reference chain on their own (x and y in one launch from the
+    // source); the others share one read of the source for their axis-0
+    // pass (gaussian_blur's first FP64 buffer, convolution.h:151-169) and
+    // then run their remaining axes through the FP64 chain.
+    std::vector<int> shared;
+    for (int l = 0; l < nlevels; l++) {
+        ndl_tensor mid0 = dense_like(src, nullptr, NDL_F64);
+        const bool pair = env_int("NDL_SCALES_PAIR", 1) &&
+                          try_xy_ref(src, nd > 2 ? &mid0 : &levels[l], 0, 1, k[l][0], k[l][1],
+                                     nd > 2 ? NDL_ROUND_TRUNC : rounding, nullptr, 1, 1, true) == 0;
+        if (pair) {
+            if ((rc = run_generic_chain(src, &levels[l], nd, axes, k[l].data(), accum, rounding, s))) return rc;
+        } else {
+            shared.push_back(l);
+        }
+    }
+    if (shared.empty()) return NDL_OK;
+    if (shared.size() == 1) {
+        const int l = shared[0];
+        return run_generic_chain(src, &levels[l], nd, axes, k[l].data(), accum, rounding, s);
+    }
     std::vector<Scratch> first(nlevels);
     std::vector<ndl_tensor> mid(nlevels);
-    for (int l = 0; l < nlevels; l++) {
+    for (int l : shared) {
         if ((rc = alloc_scratch(first[l], (size_t)n * sizeof(double), s))) return rc;
         mid[l] = dense_like(src, first[l].p, NDL_F64);
     }
+    const int ns = (int)shared.size();
     Scratch tb;
-    if ((rc = alloc_scratch(tb, (size_t)nlevels * NDL_MAX_TAPS * sizeof(double), s))) return rc;
-    std::vector<double> host((size_t)nlevels * NDL_MAX_TAPS, 0.0);
-    for (int l = 0; l < nlevels; l++) std::copy(taps[l].begin(), taps[l].end(), host.begin() + (size_t)l * NDL_MAX_TAPS);
+    if ((rc = alloc_scratch(tb, (size_t)ns * NDL_MAX_TAPS * sizeof(double), s))) return rc;
+    std::vector<double> host((size_t)ns * NDL_MAX_TAPS, 0.0);
+    for (int i = 0; i < ns; i++)
+        std::copy(taps[shared[i]].begin(), taps[shared[i]].end(), host.begin() + (size_t)i * NDL_MAX_TAPS);
     if ((rc = stage_upload(tb.p, host.data(), host.size() * sizeof(double), s))) return rc;
     MultiAxisArgs ma;
     memset(&ma, 0, sizeof ma);
@@ -2169,22 +2190,23 @@ NDL_API int ndl_gaussian_scales(const ndl_tensor* src, const ndl_tensor* levels,
         int d = order[i];
         p.ext[i] = src->extent[d];
         p.ist[i] = src->stride[d];
-        p.ost[i] = mid[0].stride[d];
+        p.ost[i] = mid[shared[0]].stride[d];
         p.fdiv[i].init((uint32_t)src->extent[d]);
         if (d == 0) p.axis_pos = i;
     }
     p.total = (uint32_t)n;
     p.border = border;
-    ma.nlev = nlevels;
+    ma.nlev = ns;
     ma.taps = (const double*)tb.p;
-    for (int l = 0; l < nlevels; l++) {
-        ma.outs[l] = first[l].p;
-        ma.K[l] = k[l][0].count;
-        ma.center[l] = k[l][0].center;
+    for (int i = 0; i < ns; i++) {
+        const int l = shared[i];
+        ma.outs[i] = first[l].p;
+        ma.K[i] = k[l][0].count;
+        ma.center[i] = k[l][0].center;
     }
     hipError_t e = launch_axis_pass_multi(ma, src->dtype, s);
     if (e != hipSuccess) return hip_fail(e, "scale-stack pass launch");
-    for (int l = 0; l < nlevels; l++)
+    for (int l : shared)
         if ((rc = run_generic_chain(&mid[l], &levels[l], nd - 1, axes + 1, k[l].data() + 1, accum, rounding, s)))
             return rc;
     return NDL_OK;
