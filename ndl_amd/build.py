@@ -20,7 +20,7 @@ BENCH_LIB = os.path.join(HERE, "libndl_benchutil.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 SOURCES = ["kernels_fused.hip", "kernels_fused_edge_a.hip", "kernels_fused_edge_b.hip", "kernels_fused_edge_c.hip",
-           "kernels_fused_edge.hip", "kernels_fused_grad.hip", "kernels_generic.hip", "kernels_line.hip", "kernels_bytes.hip", "kernels_slab.hip", "kernels_xyref.hip", "kernels_short.hip", "kernels_gradient.hip", "kernels_downsample.hip",
+           "kernels_fused_edge.hip", "kernels_fused_grad.hip", "kernels_generic.hip", "kernels_line.hip", "kernels_bytes.hip", "kernels_slab.hip", "kernels_xyref.hip", "kernels_short.hip", "kernels_gradient.hip", "kernels_downsample.hip", "kernels_wide.hip",
            "api.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",

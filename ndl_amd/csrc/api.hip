@@ -430,6 +430,65 @@ int try_short_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const 
     return e == hipSuccess ? NDL_OK : hip_fail(e, "short-axis pass launch");
 }
 
+// One pass on the wide-tap kernel (kernels_wide.hip): any axis, taps from
+// the device, no zero tap; -1 when it does not apply.  The chain tries it
+// after the one-axis kernels (odd tap counts up to 25) and the short-axis
+// kernel.
+int try_wide_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const ndl_axis_kernel& k, int accum,
+                  int rounding, hipStream_t s)
+{
+    if (!env_int("NDL_WIDE", 1)) return -1;
+    const int64_t m = in->extent[axis];
+    const int64_t n = numel(in);
+    if (n == 0 || m > 0x7fffff00LL) return -1;
+    for (int t = 0; t < k.count; t++)
+        if (k.taps[t] == 0.0) return -1;
+    WideArgs a;
+    memset(&a, 0, sizeof a);
+    a.in = in->data;
+    a.out = out->data;
+    a.m = (int32_t)m;
+    a.in_as = in->stride[axis];
+    a.out_as = out->stride[axis];
+    a.K = k.count;
+    a.center = k.center;
+    a.border = k.border;
+    a.rounding = rounding;
+    int order[NDL_MAX_DIM], no = 0;
+    for (int d = 0; d < in->ndim; d++)
+        if (d != axis) order[no++] = d;
+    std::stable_sort(order, order + no, [&](int x, int y) { return std::llabs(in->stride[x]) < std::llabs(in->stride[y]); });
+    a.nd = no;
+    for (int i = 0; i < no; i++) {
+        const int d = order[i];
+        a.ext[i] = in->extent[d];
+        a.ist[i] = in->stride[d];
+        a.ost[i] = out->stride[d];
+        a.fdiv[i].init((uint32_t)in->extent[d]);
+    }
+    const int64_t lines = n / m, chunks = (m + 7) / 8;
+    if (lines * chunks > 0xffffff00LL) return -1;
+    a.lines = (uint32_t)lines;
+    a.chunks = (uint32_t)chunks;
+    a.items = (uint32_t)(lines * chunks);
+    a.chunk_fast = std::llabs(in->stride[axis]) == 1 ? 1 : 0;
+    a.fsplit.init(a.chunk_fast ? a.chunks : a.lines);
+    Scratch tb;
+    const size_t db = (size_t)k.count * sizeof(double), fb = (size_t)k.count * sizeof(float);
+    int rc = alloc_scratch(tb, db + fb, s);
+    if (rc) return rc;
+    std::vector<unsigned char> host(db + fb);
+    memcpy(host.data(), k.taps, db);
+    float* hf = (float*)(host.data() + db);
+    for (int t = 0; t < k.count; t++) hf[t] = (float)k.taps[t];
+    if ((rc = stage_upload(tb.p, host.data(), db + fb, s))) return rc;
+    a.wd = (const double*)tb.p;
+    a.wf = (const float*)((unsigned char*)tb.p + db);
+    hipError_t e = launch_wide_axis(a, in->dtype, out->dtype, accum, s);
+    if (e == hipErrorInvalidValue) return -1;
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "wide-tap pass launch");
+}
+
 // per-axis chain through dense accumulator-typed intermediates; in
 // reference mode consecutive pairs of passes run as one launch where the
 // two-pass kernel applies
@@ -477,6 +536,7 @@ int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxe
         }
         int rc = try_one_axis(cur, out, axes[i], k[i], accum, s, 0, nullptr, last ? rounding : NDL_ROUND_TRUNC);
         if (rc == -1) rc = try_short_axis(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, s);
+        if (rc == -1) rc = try_wide_axis(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, s);
         if (rc == -1)
             rc = run_axis_pass(cur, out, axes[i], k[i], accum, last ? rounding : NDL_ROUND_TRUNC, PASS_GATHER,
                                nullptr, 0, 0, s);

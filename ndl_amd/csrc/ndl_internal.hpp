@@ -214,6 +214,31 @@ hipError_t launch_blur_downsample(const BlurDownArgs& a, int in_dt, int out_dt, 
 // register ring (kernels_line.hip).  hipErrorInvalidValue: no instance.
 hipError_t launch_blur_down_march(const BlurDownArgs& a, int accum, hipStream_t s);
 
+// ------------------------------------------------------ wide one-axis pass
+// One pass along any axis with a run-time tap count (kernels_wide.hip): a
+// thread computes 8 consecutive outputs of one line from a register window
+// of 15 elements, taps in groups of 8.  Taps on the device; no zero tap.
+struct WideArgs {
+    const void* in;
+    void* out;
+    int32_t m;                  // axis extent
+    int64_t in_as, out_as;      // axis strides
+    int32_t K, center, border, rounding;
+    int32_t nd;                 // other dimensions, fastest (by input stride) first
+    int64_t ext[NDL_MAX_DIM];
+    int64_t ist[NDL_MAX_DIM];
+    int64_t ost[NDL_MAX_DIM];
+    FastDiv fdiv[NDL_MAX_DIM];
+    uint32_t lines, chunks, items; // items = lines x chunks (8 outputs each)
+    FastDiv fsplit;             // by chunks (chunk_fast) or by lines
+    int32_t chunk_fast;         // 1: consecutive items step along the axis (unit-stride axis)
+    const double* wd;           // K taps (NDL_ACCUM_REF)
+    const float* wf;            // K taps (NDL_ACCUM_F32)
+};
+// REF: TIn -> FP64, FP64 -> TOut; F32: f32/f16/bf16 to the same type;
+// hipErrorInvalidValue: no instance
+hipError_t launch_wide_axis(const WideArgs& a, int in_dt, int out_dt, int accum, hipStream_t s);
+
 // ---------------------------------------------------------------- gradient
 // gradient(), convolution.h:359-388: every component in one launch.  One
 // thread per source element; component a is the reference's convolve() with
