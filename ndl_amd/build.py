@@ -27,6 +27,10 @@ FLAGS = [
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+# the line march's ring is a register array indexed by fully unrolled steps:
+# raise the size up to which `#pragma unroll` is honoured, or the largest tap
+# counts keep a rolled loop and put the ring in scratch memory
+EXTRA = {"kernels_line.hip": ["-mllvm", "-pragma-unroll-threshold=200000"]}
 
 
 def _includes(path, seen):
@@ -54,7 +58,7 @@ def _compile(src):
     out = os.path.join(OBJ, src.replace(".hip", ".o"))
     if not _stale(src, out):
         return out
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", out]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

@@ -966,15 +966,18 @@ bool plan_line(const ndl_tensor* in, const ndl_tensor* out, int zd, const ndl_ax
     }
     la.lanes = (uint32_t)lanes;
     // z chunks: fill the resident workgroup slots in whole rounds; a chunk
-    // of tz outputs reads tz + k - 1 slices
+    // of tz outputs reads tz + k - 1 slices in steps of whole ring periods
+    // (k + d steps, the kernel's unrolled block)
     const int64_t lblocks = (lanes + 255) / 256;
     const int64_t slots = (int64_t)device_cus() * std::max(1, line_march_blocks_per_cu(key));
+    const int period = ak.count + key.d;
     int tz = la.Z;
     double best = 1e300;
     for (int chunks = 1; chunks <= std::min(la.Z, 256); chunks++) {
         const int t = (la.Z + chunks - 1) / chunks;
         const int64_t blocks = lblocks * ((la.Z + t - 1) / t);
-        const double cost = (double)((blocks + slots - 1) / slots) * (t + ak.count - 1);
+        const int steps = (t + ak.count - 1 + period - 1) / period * period;
+        const double cost = (double)((blocks + slots - 1) / slots) * steps;
         if (cost < best - 1e-9) { best = cost; tz = t; }
     }
     const int tz_env = env_int("NDL_LINE_TZ", 0);

@@ -122,56 +122,74 @@ __global__ __launch_bounds__(256) void line_march_kernel(const LineArgs a)
     const int p0 = z0 - a.c;  // position of input 0
 
     TIn ring[R][V];
-    auto load = [&](int i, TIn* v) {
-        const int p = safe_map32(a.border, a.Z, p0 + i);
-        load_v<TIn, V>(scol + (int64_t)p * a.sz, v);
-    };
 #pragma unroll
-    for (int j = 0; j < D; j++)
-        if (j < n_in) load(j, ring[j]);
+    for (int j = 0; j < R; j++)
+#pragma unroll
+        for (int e = 0; e < V; e++) ring[j][e] = TIn{};
+    // Every step issues its load and runs its taps unconditionally (steps
+    // past the chunk re-read input n_in - 1, warm-up and trailing steps
+    // compute values nobody stores; the host sizes chunks for whole blocks
+    // of R steps): with no load or product under a branch the compiler
+    // waits for exactly the ring slot it needs, and the straight-line steps
+    // overlap the next loads with the current taps.
+    // Inputs i in [lo, hi) lie inside the axis and are read at the running
+    // offset `off` = (p0 + i) * sz; the border map runs only outside it.
+    const int lo = max(0, -p0), hi = min(n_in, a.Z - p0);
+    auto load = [&](int i, int64_t off, TIn* v) {
+        int64_t o = off;
+        if (i < lo || i >= hi) o = (int64_t)safe_map32(a.border, a.Z, p0 + min(i, n_in - 1)) * a.sz;
+        load_v<TIn, V>(scol + o, v);
+    };
+    int64_t off = (int64_t)p0 * a.sz;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        load(j, off, ring[j]);
+        off += a.sz;
+    }
+    int64_t doff = (int64_t)(z0 - (K - 1)) * a.dz; // output of step i: i - (K - 1)
     for (int i0 = 0; i0 < n_in; i0 += R) {
 #pragma unroll
         for (int u = 0; u < R; u++) {
             const int i = i0 + u;
-            if (i + D < n_in) load(i + D, ring[(u + D) % R]);
-            if (i >= K - 1 && i < n_in) {
-                TOut o[V];
-                if constexpr (OP == 0) {
-                    Acc acc[V];
+            load(i + D, off, ring[(u + D) % R]);
+            off += a.sz;
+            TOut o[V];
+            if constexpr (OP == 0) {
+                Acc acc[V];
 #pragma unroll
-                    for (int e = 0; e < V; e++) acc[e] = 0;
+                for (int e = 0; e < V; e++) acc[e] = 0;
 #pragma unroll
-                    for (int t = 0; t < K; t++)
+                for (int t = 0; t < K; t++)
 #pragma unroll
-                        for (int e = 0; e < V; e++)
-                            acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t),
-                                               to_acc<Acc>(ring[(u - (K - 1) + t + R) % R][e]));
+                    for (int e = 0; e < V; e++)
+                        acc[e] = mac1<Acc>(acc[e], tap<Acc>(a.w, a.wd, t),
+                                           to_acc<Acc>(ring[(u - (K - 1) + t + R) % R][e]));
 #pragma unroll
-                    for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e], a.rounding);
-                } else {
+                for (int e = 0; e < V; e++) from_acc<Acc>(o[e], acc[e], a.rounding);
+            } else {
 #pragma unroll
-                    for (int e = 0; e < V; e++) {
-                        TIn best = ring[(u - (K - 1) + R) % R][e];
+                for (int e = 0; e < V; e++) {
+                    TIn best = ring[(u - (K - 1) + R) % R][e];
 #pragma unroll
-                        for (int t = 1; t < K; t++) {
-                            const TIn v = ring[(u - (K - 1) + t + R) % R][e];
-                            if (morph_take<OP>(best, v)) best = v;
-                        }
-                        o[e] = best;
+                    for (int t = 1; t < K; t++) {
+                        const TIn v = ring[(u - (K - 1) + t + R) % R][e];
+                        if (morph_take<OP>(best, v)) best = v;
                     }
-                    if constexpr (std::is_floating_point<TIn>::value) {
-                        if (a.corner) {
-                            const int zc = safe_map32(a.cbz, a.Z, z0 + i - (K - 1) + a.coz);
+                    o[e] = best;
+                }
+                if constexpr (std::is_floating_point<TIn>::value) {
+                    if (a.corner && i >= K - 1 && i < n_in) {
+                        const int zc = safe_map32(a.cbz, a.Z, z0 + i - (K - 1) + a.coz);
 #pragma unroll
-                            for (int e = 0; e < V; e++) {
-                                const TIn cv = ((const TIn*)a.corner)[cxo[e] + (int64_t)zc * a.czs];
-                                if (cv != cv) o[e] = cv;
-                            }
+                        for (int e = 0; e < V; e++) {
+                            const TIn cv = ((const TIn*)a.corner)[cxo[e] + (int64_t)zc * a.czs];
+                            if (cv != cv) o[e] = cv;
                         }
                     }
                 }
-                if (live) store_v<TOut, V>(dcol + (int64_t)(z0 + i - (K - 1)) * a.dz, o);
             }
+            if (live && i >= K - 1 && i < n_in) store_v<TOut, V>(dcol + doff, o);
+            doff += a.dz;
         }
     }
 }
