@@ -2407,6 +2407,13 @@ NDL_API int ndl_gaussian_downsample(const ndl_tensor* src, const ndl_tensor* dst
     const int64_t lane_blocks = march ? ((int64_t)a.lanes + 4 * a.ncol - 1) / (4 * a.ncol) : ((int64_t)a.lanes + 255) / 256;
     const int64_t want_chunks = std::max<int64_t>(1, 8 * (int64_t)device_cus() / lane_blocks);
     a.tz = (int32_t)std::max<int64_t>(1, std::min<int64_t>(oz, std::max<int64_t>(4, (oz + want_chunks - 1) / want_chunks)));
+    if (march) {
+        // the march runs whole ring periods of k + 8 steps: grow the chunk
+        // to the most outputs that the same number of periods holds
+        const int f = a.axis == channel_axis ? 1 : factor, period = lk.count + 8;
+        const int64_t steps = ((int64_t)a.tz * f + lk.count - 1 + period - 1) / period * period;
+        a.tz = (int32_t)std::max<int64_t>(a.tz, std::min<int64_t>(oz, (steps - (lk.count - 1)) / f));
+    }
     a.tiles_z = (int32_t)((oz + a.tz - 1) / a.tz);
     Scratch wide; // released (stream-ordered) after the launch
     if (lk.count <= kArgTaps) {

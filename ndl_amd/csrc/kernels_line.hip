@@ -498,22 +498,37 @@ __global__ __launch_bounds__(256) void blur_down_march_kernel(const BlurDownArgs
     const int64_t sz = a.ist[a.axis], dz = a.ost[a.axis];
     const int src_lane = ci;
 
+    // as line_march_kernel: every step loads and runs its taps
+    // unconditionally (past the chunk it re-reads input n_in - 1), so the
+    // compiler waits for one ring slot at a time; inputs in [lo, hi) are
+    // read at a running offset, the border map runs outside it
     Acc ring[R];
-    auto load = [&](int i) { return scol[(int64_t)safe_map32(a.border, Z, p0 + i) * sz]; };
 #pragma unroll
-    for (int j = 0; j < D; j++)
-        if (j < n_in) ring[j] = load(j);
+    for (int j = 0; j < R; j++) ring[j] = Acc(0);
+    const int lo = max(0, -p0), hi = min(n_in, Z - p0);
+    auto load = [&](int i, int64_t off) {
+        int64_t o = off;
+        if (i < lo || i >= hi) o = (int64_t)safe_map32(a.border, Z, p0 + min(i, n_in - 1)) * sz;
+        return scol[o];
+    };
+    int64_t off = (int64_t)p0 * sz;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        ring[j] = load(j, off);
+        off += sz;
+    }
     double sum = 0.0;
     int nz = 0, ob = ob0;
     for (int i0 = 0; i0 < n_in; i0 += R) {
 #pragma unroll
         for (int u = 0; u < R; u++) {
             const int i = i0 + u;
-            if (i + D < n_in) ring[(u + D) % R] = load(i + D);
-            if (i >= K - 1 && i < n_in) {
-                Acc acc = 0;
+            ring[(u + D) % R] = load(i + D, off);
+            off += sz;
+            Acc acc = 0;
 #pragma unroll
-                for (int t = 0; t < K; t++) acc = mac1<Acc>(acc, bd_tap<Acc>(a, t), ring[(u - (K - 1) + t + R) % R]);
+            for (int t = 0; t < K; t++) acc = mac1<Acc>(acc, bd_tap<Acc>(a, t), ring[(u - (K - 1) + t + R) % R]);
+            if (i >= K - 1 && i < n_in) {
                 const double v = stored_value<Acc>(acc, a.out_dt, a.rounding);
                 const int ok = valid ? 1 : 0;
                 for (int q = 0; q < a.P; q++) {
