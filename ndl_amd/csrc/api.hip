@@ -331,6 +331,7 @@ int try_one_axis(const ndl_tensor* in, const ndl_tensor* out, int axis, const nd
 // downsampled image (block means over fx x fy, see XYRefArgs); dry: only
 // whether the kernel applies (0) or not (-1).
 int env_int(const char* name, int dflt);
+int device_cus();
 int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, const ndl_axis_kernel& kx,
                const ndl_axis_kernel& ky, int rounding, hipStream_t s, int fx = 1, int fy = 1, bool dry = false)
 {
@@ -370,17 +371,27 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
     a.tiles_y = (a.Y + kXYTileY - 1) / kXYTileY;
     a.ftx.init((uint32_t)a.tiles_x);
     a.fty.init((uint32_t)a.tiles_y);
-    int64_t tiles = (int64_t)a.tiles_x * a.tiles_y;
+    int64_t tiles = (int64_t)a.tiles_x * a.tiles_y, batch = 1;
     for (int d = 0; d < in->ndim; d++) {
         if (d == ax || d == ay) continue;
         a.bext[a.nb].init((uint32_t)std::min<int64_t>(in->extent[d], 0xffffffffLL));
         a.ib[a.nb] = in->stride[d];
         a.ob[a.nb++] = out->stride[d];
-        tiles *= in->extent[d];
-        if (tiles > 0x7fffffffLL) return -1;
+        batch *= in->extent[d];
+        if (tiles * batch > 0x7fffffffLL) return -1;
     }
-    if (dry || tiles == 0) return NDL_OK;
-    a.ntiles = (uint32_t)tiles;
+    if (dry || tiles * batch == 0) return NDL_OK;
+    // a workgroup runs nsl consecutive indices of the other dimensions (the
+    // next window is read while the current one is filtered); the launch
+    // keeps about four workgroups per CU
+    int64_t nsl = std::max<int64_t>(1, tiles * batch / (4 * (int64_t)device_cus()));
+    nsl = std::min<int64_t>(nsl, 16);
+    const int nsl_env = env_int("NDL_XY_SLICES", 0);
+    if (nsl_env > 0) nsl = nsl_env;
+    nsl = std::min(nsl, batch);
+    a.nbatch = (uint32_t)batch;
+    a.nsl = (uint32_t)nsl;
+    a.ntiles = (uint32_t)(tiles * ((batch + nsl - 1) / nsl));
     hipError_t e = launch_xy_ref(a, it, ot, s);
     if (e == hipErrorInvalidValue) return -1;
     return e == hipSuccess ? NDL_OK : hip_fail(e, "two-pass reference launch");

@@ -10,8 +10,9 @@
 // holds; the y pass result is rounded once into the output type.
 // Bit-identical to the two passes of the chain.
 //
-// A workgroup (4 waves) owns a 64 x 32 tile of outputs (x by y) of one index
-// of the other dimensions:
+// A workgroup (4 waves) owns a 64 x 32 tile of outputs (x by y) of nsl
+// consecutive indices of the other dimensions, one after the other (the
+// window of the next index is requested before the current one is filtered):
 //  1. the input window, (32 + K - 1) rows of (64 + K - 1) elements with the
 //     border folded per coordinate, is read into LDS in the input type (wave
 //     w takes rows w, w + 4, ...; a lane one element, two when the row is
@@ -85,40 +86,55 @@ __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
     t = q;
     q = a.fty.div(t);
     const int y0 = (int)(t - q * (uint32_t)a.tiles_y) * TYH;
-    t = q;
-    int64_t ib = 0, ob = 0;
-    for (int i = 0; i < a.nb; i++) {
-        q = a.bext[i].div(t);
-        const uint32_t r = t - q * a.bext[i].d;
-        ib += (int64_t)r * a.ib[i];
-        ob += (int64_t)r * a.ob[i];
-        t = q;
-    }
-    const TIn* in = (const TIn*)a.in + ib;
+    // this workgroup's indices of the other dimensions: b0 .. b1-1 of
+    // their flattened range (dimension order as listed by the host)
+    const uint32_t b0 = q * a.nsl, b1 = min(b0 + a.nsl, a.nbatch);
+    auto batch_off = [&](uint32_t bb, int64_t& ib, int64_t& ob) {
+        ib = 0;
+        ob = 0;
+        for (int i = 0; i < a.nb; i++) {
+            const uint32_t qq = a.bext[i].div(bb);
+            const uint32_t r = bb - qq * a.bext[i].d;
+            ib += (int64_t)r * a.ib[i];
+            ob += (int64_t)r * a.ob[i];
+            bb = qq;
+        }
+    };
 
-    // 1. input window: every load of the wave is issued before the first
-    //    LDS store (rows are compile-time register slots).  Lanes past the
-    //    window's right end repeat its last element (same address, same
-    //    value), so no load or store is predicated per lane; rows fold only
-    //    in tiles that reach past the first or last row.
+    // 1. input window: lanes past the window's right end repeat its last
+    //    element (same address, same value), so no load or store is
+    //    predicated per lane; rows fold only in tiles that reach past the
+    //    first or last row.  Column and row offsets are the same for every
+    //    index of the other dimensions: computed once, and the window of the
+    //    next index is requested while the current one is filtered.
+    constexpr int RW = (HA + 3) / 4; // window rows per wave, at most
+    const int64_t xo0 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + lane) * a.isx;
+    const int c1 = min(lane + 64, WA - 1);
+    const int64_t xo1 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + c1) * a.isx;
+    int64_t roff[RW];
     {
-        constexpr int RW = (HA + 3) / 4; // window rows per wave, at most
-        const int64_t xo0 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + lane) * a.isx;
-        const int c1 = min(lane + 64, WA - 1);
-        const int64_t xo1 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + c1) * a.isx;
         const int ybase = y0 - a.cy;
         const bool yin = ybase >= 0 && ybase + HA <= a.Y;
-        TIn e0[RW], e1[RW];
 #pragma unroll
         for (int j = 0; j < RW; j++) {
-            const int r = wv + 4 * j;
-            if (r < HA) {
-                const int yy = yin ? ybase + r : safe_map32(a.by, a.Y, ybase + r);
-                const TIn* row = in + (int64_t)yy * a.isy;
-                e0[j] = row[xo0];
-                e1[j] = row[xo1];
+            const int r = min(wv + 4 * j, HA - 1);
+            roff[j] = (int64_t)(yin ? ybase + r : safe_map32(a.by, a.Y, ybase + r)) * a.isy;
+        }
+    }
+    TIn e0[RW], e1[RW];
+    auto issue = [&](const TIn* in) {
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            if (wv + 4 * j < HA) {
+                e0[j] = in[roff[j] + xo0];
+                e1[j] = in[roff[j] + xo1];
             }
         }
+    };
+    int64_t ib, ob;
+    batch_off(b0, ib, ob);
+    issue((const TIn*)a.in + ib);
+    for (uint32_t bb = b0; bb < b1; bb++) {
 #pragma unroll
         for (int j = 0; j < RW; j++) {
             const int r = wv + 4 * j;
@@ -127,52 +143,57 @@ __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
                 win[r * WA + c1] = e1[j];
             }
         }
-    }
-    __syncthreads();
-
-    // 2. x pass: task (window row r, outputs 4s .. 4s+3)
-    for (int r = threadIdx.x >> 4; r < HA; r += 16) {
-        const int s = threadIdx.x & 15;
-        const TIn* src = win + r * WA + 4 * s;
-        double v[K + 3];
-#pragma unroll
-        for (int i = 0; i < K + 3; i++) v[i] = load_f64(src + i);
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const double w = a.wx[k];
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+        __syncthreads();
+        const int64_t obc = ob;
+        if (bb + 1 < b1) {
+            batch_off(bb + 1, ib, ob);
+            issue((const TIn*)a.in + ib);
         }
-#pragma unroll
-        for (int j = 0; j < 4; j++) plane[r * TXW + 4 * s + j] = acc[j];
-    }
-    __syncthreads();
 
-    // 3. y pass: task (column lane, rows 4s .. 4s+3)
-    const int x = x0 + lane;
-    for (int s = wv; s < TYH / 4; s += 4) {
-        const double* src = plane + 4 * s * TXW + lane;
-        double v[K + 3];
+        // 2. x pass: task (window row r, outputs 4s .. 4s+3)
+        for (int r = threadIdx.x >> 4; r < HA; r += 16) {
+            const int s = threadIdx.x & 15;
+            const TIn* src = win + r * WA + 4 * s;
+            double v[K + 3];
 #pragma unroll
-        for (int i = 0; i < K + 3; i++) v[i] = src[i * TXW];
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int i = 0; i < K + 3; i++) v[i] = load_f64(src + i);
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const double w = a.wy[k];
+            for (int k = 0; k < K; k++) {
+                const double w = a.wx[k];
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+                for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) plane[r * TXW + 4 * s + j] = acc[j];
         }
-        if constexpr (DEC) {
-            if (a.fy == 1) store_block_means<TOut, 1>(a, acc, x0, y0 + 4 * s, lane, ob);
-            else if (a.fy == 2) store_block_means<TOut, 2>(a, acc, x0, y0 + 4 * s, lane, ob);
-            else store_block_means<TOut, 4>(a, acc, x0, y0 + 4 * s, lane, ob);
-        } else if (x < a.X) {
-            TOut* col = (TOut*)a.out + ob + (int64_t)x * a.osx;
+        __syncthreads();
+
+        // 3. y pass: task (column lane, rows 4s .. 4s+3)
+        const int x = x0 + lane;
+        for (int s = wv; s < TYH / 4; s += 4) {
+            const double* src = plane + 4 * s * TXW + lane;
+            double v[K + 3];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int y = y0 + 4 * s + j;
-                if (y < a.Y) store_from_f64(col + (int64_t)y * a.osy, acc[j], a.rounding);
+            for (int i = 0; i < K + 3; i++) v[i] = src[i * TXW];
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double w = a.wy[k];
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+            }
+            if constexpr (DEC) {
+                if (a.fy == 1) store_block_means<TOut, 1>(a, acc, x0, y0 + 4 * s, lane, obc);
+                else if (a.fy == 2) store_block_means<TOut, 2>(a, acc, x0, y0 + 4 * s, lane, obc);
+                else store_block_means<TOut, 4>(a, acc, x0, y0 + 4 * s, lane, obc);
+            } else if (x < a.X) {
+                TOut* col = (TOut*)a.out + obc + (int64_t)x * a.osx;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int y = y0 + 4 * s + j;
+                    if (y < a.Y) store_from_f64(col + (int64_t)y * a.osy, acc[j], a.rounding);
+                }
             }
         }
     }

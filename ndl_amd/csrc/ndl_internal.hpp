@@ -277,7 +277,8 @@ struct XYRefArgs {
     int32_t nb;                 // other dimensions
     FastDiv bext[NDL_MAX_DIM];
     int64_t ib[NDL_MAX_DIM], ob[NDL_MAX_DIM];
-    uint32_t ntiles;
+    uint32_t ntiles;            // tiles_x * tiles_y * ceil(nbatch / nsl)
+    uint32_t nbatch, nsl;       // indices of the other dimensions; per workgroup
     double wx[kXYMaxTaps], wy[kXYMaxTaps];
     // fx * fy > 1: store downsample()'s block means of the y pass results
     // (rounded as the blur stores them) over fx x fy blocks (1, 2 or 4 each),
