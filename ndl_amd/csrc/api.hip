@@ -397,6 +397,80 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
     return e == hipSuccess ? NDL_OK : hip_fail(e, "two-pass reference launch");
 }
 
+// Three consecutive reference passes in one launch (kernels_xyref.hip): a
+// first axis ac of at most 4 elements (colour channels), then ax, then ay;
+// -1 when the kernel does not apply: unequal tap counts or counts other than
+// odd 3..9 (a first axis of at most 4 elements takes no wider kernel), a
+// zero tap, a longer first axis, an x extent under 16 or a y
+// extent under 8, element types other than uint8, uint16, int16 and float32
+// (the output FP64 or the input's type), more LDS than a workgroup may take,
+// or more than 2^31 tiles.
+int try_cxy_ref(const ndl_tensor* in, const ndl_tensor* out, int ac, int ax, int ay, const ndl_axis_kernel& kc,
+                const ndl_axis_kernel& kx, const ndl_axis_kernel& ky, int rounding, hipStream_t s)
+{
+    if (!env_int("NDL_CXY_REF", 1) || !env_int("NDL_XY_REF", 1)) return -1;
+    if (ac == ax || ac == ay || ax == ay) return -1;
+    if (in->extent[ac] < 1 || in->extent[ac] > kCXYMaxC) return -1;
+    if (kc.count != kx.count || kx.count != ky.count || kx.count > 9 || kx.count % 2 == 0) return -1;
+    for (int t = 0; t < kx.count; t++)
+        if (kc.taps[t] == 0.0 || kx.taps[t] == 0.0 || ky.taps[t] == 0.0) return -1;
+    if (in->extent[ax] < 16 || in->extent[ay] < 8) return -1;
+    if (in->extent[ax] > 0x7fffffffLL || in->extent[ay] > 0x7fffffffLL) return -1;
+    const int it = in->dtype, ot = out->dtype;
+    if (it != NDL_U8 && it != NDL_U16 && it != NDL_I16 && it != NDL_F32) return -1;
+    if (ot != it && ot != NDL_F64) return -1;
+    if (cxy_ref_lds_bytes(kx.count, (int)dtype_size(it)) > 65536) return -1;
+    CXYRefArgs ca;
+    memset(&ca, 0, sizeof ca);
+    XYRefArgs& a = ca.xy;
+    a.in = in->data;
+    a.out = out->data;
+    a.X = (int32_t)in->extent[ax];
+    a.Y = (int32_t)in->extent[ay];
+    a.isx = in->stride[ax];
+    a.isy = in->stride[ay];
+    a.osx = out->stride[ax];
+    a.osy = out->stride[ay];
+    a.kx = kx.count;
+    a.cx = kx.center;
+    a.bx = kx.border;
+    a.ky = ky.count;
+    a.cy = ky.center;
+    a.by = ky.border;
+    for (int t = 0; t < kx.count; t++) {
+        a.wx[t] = kx.taps[t];
+        a.wy[t] = ky.taps[t];
+        ca.wc[t] = kc.taps[t];
+    }
+    a.rounding = rounding;
+    a.fx = a.fy = 1;
+    ca.m = (int32_t)in->extent[ac];
+    ca.ics = in->stride[ac];
+    ca.ocs = out->stride[ac];
+    ca.cc = kc.center;
+    ca.cb = kc.border;
+    a.tiles_x = (a.X + kXYTileX - 1) / kXYTileX;
+    a.tiles_y = (a.Y + kXYTileY - 1) / kXYTileY;
+    a.ftx.init((uint32_t)a.tiles_x);
+    a.fty.init((uint32_t)a.tiles_y);
+    int64_t tiles = (int64_t)a.tiles_x * a.tiles_y;
+    for (int d = 0; d < in->ndim; d++) {
+        if (d == ac || d == ax || d == ay) continue;
+        a.bext[a.nb].init((uint32_t)std::min<int64_t>(in->extent[d], 0xffffffffLL));
+        a.ib[a.nb] = in->stride[d];
+        a.ob[a.nb++] = out->stride[d];
+        tiles *= in->extent[d];
+        if (tiles > 0x7fffffffLL) return -1;
+    }
+    if (tiles == 0) return NDL_OK;
+    a.ntiles = (uint32_t)tiles;
+    a.nbatch = (uint32_t)(tiles / ((int64_t)a.tiles_x * a.tiles_y));
+    a.nsl = 1;
+    hipError_t e = launch_cxy_ref(ca, it, ot, s);
+    if (e == hipErrorInvalidValue) return -1;
+    return e == hipSuccess ? NDL_OK : hip_fail(e, "three-pass reference launch");
+}
+
 // One pass along an axis of at most 16 elements on the short-axis kernel
 // (kernels_short.hip); -1 when it does not apply (a longer axis, more than
 // 64 taps, 2^32 lines or more, or element types without an instance).
@@ -522,6 +596,24 @@ int run_generic_chain(const ndl_tensor* src, const ndl_tensor* dst, int32_t naxe
     const ndl_tensor* cur = src;
     int slot = 0;
     for (int i = 0; i < naxes;) {
+        if (accum == NDL_ACCUM_REF && i + 3 <= naxes) {
+            // a short first axis (colour channels) and the two passes after it
+            const bool last = i + 3 == naxes;
+            const ndl_tensor* out = dst;
+            if (!last) {
+                int rc = inter(slot, out);
+                if (rc) return rc;
+            }
+            const int rc = try_cxy_ref(cur, out, axes[i], axes[i + 1], axes[i + 2], k[i], k[i + 1], k[i + 2],
+                                       last ? rounding : NDL_ROUND_TRUNC, s);
+            if (rc != -1) {
+                if (rc) return rc;
+                cur = out;
+                slot ^= 1;
+                i += 3;
+                continue;
+            }
+        }
         if (accum == NDL_ACCUM_REF && i + 1 < naxes) {
             const bool last = i + 2 == naxes;
             const ndl_tensor* out = dst;

@@ -199,6 +199,154 @@ __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
     }
 }
 
+// The same two passes preceded by a pass along a short axis c (at most four
+// elements: the channels of a colour image), all three in the reference's
+// order and arithmetic (the chain's c pass writes an FP64 image, x and y
+// read it).  The tile loads the raw window of every channel once; for each
+// output channel it forms the c-pass values of the window in an FP64 LDS
+// window (taps ascending, the short-axis kernel's border rule: position
+// ch + t - cc mapped once, a position past the axis reading element 0, a
+// mapped position 1..3 beyond the extent reading zero), then runs the x and
+// y passes of xy_ref_kernel on it.  No FP64 intermediate reaches HBM.
+template <class TIn, class TOut, int K>
+__global__ __launch_bounds__(256) void cxy_ref_kernel(const CXYRefArgs ca)
+{
+    extern __shared__ __align__(16) unsigned char cxy_lds[];
+    const XYRefArgs& a = ca.xy;
+    constexpr int WA = TXW + K - 1, HA = TYH + K - 1;
+    double* plane = (double*)cxy_lds;                                       // HA x TXW
+    double* cw = plane + HA * TXW;                                          // HA x WA, c-pass values
+    TIn* win = (TIn*)(cxy_lds + (size_t)(HA * TXW + HA * WA) * sizeof(double)); // m x HA x WA, raw
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = ca.m;
+
+    uint32_t t = blockIdx.x;
+    uint32_t q = a.ftx.div(t);
+    const int x0 = (int)(t - q * (uint32_t)a.tiles_x) * TXW;
+    t = q;
+    q = a.fty.div(t);
+    const int y0 = (int)(t - q * (uint32_t)a.tiles_y) * TYH;
+    t = q;
+    int64_t ib = 0, ob = 0;
+    for (int i = 0; i < a.nb; i++) {
+        q = a.bext[i].div(t);
+        const uint32_t r = t - q * a.bext[i].d;
+        ib += (int64_t)r * a.ib[i];
+        ob += (int64_t)r * a.ob[i];
+        t = q;
+    }
+
+    // 1. raw windows of every channel (as xy_ref_kernel: lanes past the right
+    //    end repeat the last element, rows fold only near the first or last row)
+    {
+        constexpr int RW = (HA + 3) / 4;
+        const int64_t xo0 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + lane) * a.isx;
+        const int c1 = min(lane + 64, WA - 1);
+        const int64_t xo1 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + c1) * a.isx;
+        const int ybase = y0 - a.cy;
+        const bool yin = ybase >= 0 && ybase + HA <= a.Y;
+        for (int ch = 0; ch < m; ch++) {
+            const TIn* in = (const TIn*)a.in + ib + (int64_t)ch * ca.ics;
+            TIn* w = win + ch * (HA * WA);
+            TIn e0[RW], e1[RW];
+#pragma unroll
+            for (int j = 0; j < RW; j++) {
+                const int r = wv + 4 * j;
+                if (r < HA) {
+                    const int yy = yin ? ybase + r : safe_map32(a.by, a.Y, ybase + r);
+                    const TIn* row = in + (int64_t)yy * a.isy;
+                    e0[j] = row[xo0];
+                    e1[j] = row[xo1];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < RW; j++) {
+                const int r = wv + 4 * j;
+                if (r < HA) {
+                    w[r * WA + lane] = e0[j];
+                    w[r * WA + c1] = e1[j];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int x = x0 + lane;
+    for (int ch = 0; ch < m; ch++) {
+        // 1b. c pass of output channel ch over the window; the source channel
+        //     of every tap is the same for the whole workgroup
+        int soff[K];
+        bool zero[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = border_map32(ca.cb, m, ch + k - ca.cc);
+            const bool mid = e >= 1 && e <= 3;
+            soff[k] = mid && e < m ? e * (HA * WA) : 0;
+            zero[k] = mid && e >= m;
+        }
+        for (int idx = threadIdx.x; idx < HA * WA; idx += 256) {
+            TIn raw[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) raw[k] = win[soff[k] + idx];
+            double total = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double v = zero[k] ? 0.0 : load_f64(&raw[k]);
+                total = total + v * ca.wc[k];
+            }
+            cw[idx] = total;
+        }
+        __syncthreads();
+
+        // 2. x pass: task (window row r, outputs 4s .. 4s+3)
+        for (int r = threadIdx.x >> 4; r < HA; r += 16) {
+            const int s = threadIdx.x & 15;
+            const double* src = cw + r * WA + 4 * s;
+            double v[K + 3];
+#pragma unroll
+            for (int i = 0; i < K + 3; i++) v[i] = src[i];
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double w = a.wx[k];
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) plane[r * TXW + 4 * s + j] = acc[j];
+        }
+        __syncthreads();
+
+        // 3. y pass: task (column lane, rows 4s .. 4s+3), stored into channel ch
+        for (int s = wv; s < TYH / 4; s += 4) {
+            const double* src = plane + 4 * s * TXW + lane;
+            double v[K + 3];
+#pragma unroll
+            for (int i = 0; i < K + 3; i++) v[i] = src[i * TXW];
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double w = a.wy[k];
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
+            }
+            if (x < a.X) {
+                TOut* col = (TOut*)a.out + ob + (int64_t)ch * ca.ocs + (int64_t)x * a.osx;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int y = y0 + 4 * s + j;
+                    if (y < a.Y) store_from_f64(col + (int64_t)y * a.osy, acc[j], a.rounding);
+                }
+            }
+        }
+        // the next channel's c pass rewrites cw only after every wave has
+        // passed the barrier behind the x pass; its x pass rewrites the
+        // plane only after the barrier behind that c pass, which every wave
+        // reaches after its y pass
+    }
+}
+
 template <int V> using IC = std::integral_constant<int, V>;
 
 // f(TIn, TOut, K): uint8 / uint16 / int16 / float32 / float64 input, the
@@ -244,6 +392,45 @@ size_t xy_ref_lds_bytes(int kx, int ky, int in_size)
 {
     const size_t ha = (size_t)(TYH + ky - 1);
     return ha * TXW * sizeof(double) + ha * (size_t)(TXW + kx - 1) * (size_t)in_size;
+}
+
+size_t cxy_ref_lds_bytes(int k, int in_size)
+{
+    const size_t ha = (size_t)(TYH + k - 1), wa = (size_t)(TXW + k - 1);
+    return ha * TXW * sizeof(double) + ha * wa * sizeof(double) + (size_t)kCXYMaxC * ha * wa * (size_t)in_size;
+}
+
+hipError_t launch_cxy_ref(const CXYRefArgs& a, int in_dt, int out_dt, hipStream_t s)
+{
+    if (a.xy.ntiles == 0) return hipSuccess;
+    if (a.xy.kx != a.xy.ky || a.m < 1 || a.m > kCXYMaxC || a.xy.kx > 9) return hipErrorInvalidValue;
+    if (out_dt != in_dt && out_dt != NDL_F64) return hipErrorInvalidValue;
+    auto go = [&](auto tin, auto tout) -> hipError_t {
+        using TIn = decltype(tin);
+        using TOut = decltype(tout);
+        const size_t lds = cxy_ref_lds_bytes(a.xy.kx, (int)sizeof(TIn));
+        if (lds > 65536) return hipErrorInvalidValue;
+        switch (a.xy.kx) {
+#define NDL_K(KK)                                                                                              \
+    case KK:                                                                                                   \
+        hipLaunchKernelGGL((cxy_ref_kernel<TIn, TOut, KK>), dim3(a.xy.ntiles), dim3(256), lds, s, a);          \
+        return hipGetLastError();
+            NDL_K(3) NDL_K(5) NDL_K(7) NDL_K(9)
+#undef NDL_K
+        }
+        return hipErrorInvalidValue;
+    };
+    auto pick = [&](auto tin) -> hipError_t {
+        if (out_dt == NDL_F64) return go(tin, double{});
+        return go(tin, tin);
+    };
+    switch (in_dt) {
+    case NDL_U8: return pick(uint8_t{});
+    case NDL_U16: return pick(uint16_t{});
+    case NDL_I16: return pick(int16_t{});
+    case NDL_F32: return pick(float{});
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_xy_ref(const XYRefArgs& a, int in_dt, int out_dt, hipStream_t s)

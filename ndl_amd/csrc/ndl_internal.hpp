@@ -287,6 +287,22 @@ struct XYRefArgs {
 };
 // LDS bytes of one workgroup for these tap counts and input type size
 size_t xy_ref_lds_bytes(int kx, int ky, int in_size);
+// Three reference passes in one launch: a short first axis c of at most
+// kCXYMaxC elements (colour channels), then x, then y (XYRefArgs for x / y;
+// the tile runs every index of c).  Taps of c: wc, cc, cb; one tap count K
+// for all three axes.
+constexpr int kCXYMaxC = 4;
+struct CXYRefArgs {
+    XYRefArgs xy;
+    int32_t m;                  // extent of c
+    int64_t ics, ocs;           // strides of c (elements)
+    int32_t cc, cb;             // centre and border of the c taps
+    double wc[kXYMaxTaps];
+};
+size_t cxy_ref_lds_bytes(int k, int in_size);
+// out_dt: NDL_F64 or in_dt (uint8 / uint16 / int16 / float32 input);
+// hipErrorInvalidValue: no instance or more LDS than a workgroup may take
+hipError_t launch_cxy_ref(const CXYRefArgs& a, int in_dt, int out_dt, hipStream_t s);
 // out_dt: NDL_F64 or in_dt; hipErrorInvalidValue: no instance
 hipError_t launch_xy_ref(const XYRefArgs& a, int in_dt, int out_dt, hipStream_t s);
 
