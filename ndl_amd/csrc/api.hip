@@ -406,7 +406,8 @@ int try_xy_ref(const ndl_tensor* in, const ndl_tensor* out, int ax, int ay, cons
 // (the output FP64 or the input's type), more LDS than a workgroup may take,
 // or more than 2^31 tiles.
 int try_cxy_ref(const ndl_tensor* in, const ndl_tensor* out, int ac, int ax, int ay, const ndl_axis_kernel& kc,
-                const ndl_axis_kernel& kx, const ndl_axis_kernel& ky, int rounding, hipStream_t s)
+                const ndl_axis_kernel& kx, const ndl_axis_kernel& ky, int rounding, hipStream_t s, int fx = 1,
+                int fy = 1, bool dry = false)
 {
     if (!env_int("NDL_CXY_REF", 1) || !env_int("NDL_XY_REF", 1)) return -1;
     if (ac == ax || ac == ay || ax == ay) return -1;
@@ -418,7 +419,7 @@ int try_cxy_ref(const ndl_tensor* in, const ndl_tensor* out, int ac, int ax, int
     if (in->extent[ax] > 0x7fffffffLL || in->extent[ay] > 0x7fffffffLL) return -1;
     const int it = in->dtype, ot = out->dtype;
     if (it != NDL_U8 && it != NDL_U16 && it != NDL_I16 && it != NDL_F32) return -1;
-    if (ot != it && ot != NDL_F64) return -1;
+    if (ot != it && (ot != NDL_F64 || fx * fy > 1)) return -1;
     if (cxy_ref_lds_bytes(kx.count, (int)dtype_size(it)) > 65536) return -1;
     CXYRefArgs ca;
     memset(&ca, 0, sizeof ca);
@@ -443,7 +444,8 @@ int try_cxy_ref(const ndl_tensor* in, const ndl_tensor* out, int ac, int ax, int
         ca.wc[t] = kc.taps[t];
     }
     a.rounding = rounding;
-    a.fx = a.fy = 1;
+    a.fx = fx;
+    a.fy = fy;
     ca.m = (int32_t)in->extent[ac];
     ca.ics = in->stride[ac];
     ca.ocs = out->stride[ac];
@@ -462,7 +464,7 @@ int try_cxy_ref(const ndl_tensor* in, const ndl_tensor* out, int ac, int ax, int
         tiles *= in->extent[d];
         if (tiles > 0x7fffffffLL) return -1;
     }
-    if (tiles == 0) return NDL_OK;
+    if (dry || tiles == 0) return NDL_OK;
     a.ntiles = (uint32_t)tiles;
     a.nbatch = (uint32_t)(tiles / ((int64_t)a.tiles_x * a.tiles_y));
     a.nsl = 1;
@@ -2437,6 +2439,19 @@ NDL_API int ndl_gaussian_downsample(const ndl_tensor* src, const ndl_tensor* dst
     // reference arithmetic: the last two blurred passes and the block means
     // in one launch (kernels_xyref.hip) when the two axes are the only
     // decimated ones, with factors 2 or 4 and near-unit-stride rows
+    // a colour image: the channel pass as well, when the channel axis is the
+    // first blurred one and has at most 4 elements (kernels_xyref.hip)
+    if (accum == NDL_ACCUM_REF && nb == 3 && axes[0] == channel_axis && env_int("NDL_XY_DOWN", 1)) {
+        const int ax = axes[1], ay = axes[2];
+        bool ok = ax < ay && (factor == 2 || factor == 4);
+        for (int d = 0; d < nd; d++)
+            if (d != ax && d != ay && d != channel_axis && src->extent[d] > 1) ok = false;
+        if (ok && std::llabs(src->stride[ax]) <= 4 &&
+            try_cxy_ref(src, dst, axes[0], ax, ay, k[0], k[1], k[2], rounding, nullptr, factor, factor, true) == 0) {
+            rc = try_cxy_ref(src, dst, axes[0], ax, ay, k[0], k[1], k[2], rounding, s, factor, factor);
+            return rc != -1 ? rc : fail(NDL_ERR_ARG, "no three-pass instance for an accepted blur-downsample");
+        }
+    }
     if (accum == NDL_ACCUM_REF && nb >= 2 && env_int("NDL_XY_DOWN", 1)) {
         const int ax = axes[nb - 2], ay = axes[nb - 1];
         const int fx = ax == channel_axis ? 1 : factor, fy = ay == channel_axis ? 1 : factor;

@@ -207,8 +207,10 @@ __global__ __launch_bounds__(256) void xy_ref_kernel(const XYRefArgs a)
 // window (taps ascending, the short-axis kernel's border rule: position
 // ch + t - cc mapped once, a position past the axis reading element 0, a
 // mapped position 1..3 beyond the extent reading zero), then runs the x and
-// y passes of xy_ref_kernel on it.  No FP64 intermediate reaches HBM.
-template <class TIn, class TOut, int K>
+// y passes of xy_ref_kernel on it.  No FP64 intermediate reaches HBM.  DEC:
+// the y pass results go to downsample()'s block means over x and y as in
+// xy_ref_kernel (channels are not decimated).
+template <class TIn, class TOut, int K, bool DEC>
 __global__ __launch_bounds__(256) void cxy_ref_kernel(const CXYRefArgs ca)
 {
     extern __shared__ __align__(16) unsigned char cxy_lds[];
@@ -331,8 +333,13 @@ __global__ __launch_bounds__(256) void cxy_ref_kernel(const CXYRefArgs ca)
 #pragma unroll
                 for (int j = 0; j < 4; j++) acc[j] = acc[j] + v[k + j] * w;
             }
-            if (x < a.X) {
-                TOut* col = (TOut*)a.out + ob + (int64_t)ch * ca.ocs + (int64_t)x * a.osx;
+            const int64_t obch = ob + (int64_t)ch * ca.ocs;
+            if constexpr (DEC) {
+                if (a.fy == 1) store_block_means<TOut, 1>(a, acc, x0, y0 + 4 * s, lane, obch);
+                else if (a.fy == 2) store_block_means<TOut, 2>(a, acc, x0, y0 + 4 * s, lane, obch);
+                else store_block_means<TOut, 4>(a, acc, x0, y0 + 4 * s, lane, obch);
+            } else if (x < a.X) {
+                TOut* col = (TOut*)a.out + obch + (int64_t)x * a.osx;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const int y = y0 + 4 * s + j;
@@ -405,6 +412,8 @@ hipError_t launch_cxy_ref(const CXYRefArgs& a, int in_dt, int out_dt, hipStream_
     if (a.xy.ntiles == 0) return hipSuccess;
     if (a.xy.kx != a.xy.ky || a.m < 1 || a.m > kCXYMaxC || a.xy.kx > 9) return hipErrorInvalidValue;
     if (out_dt != in_dt && out_dt != NDL_F64) return hipErrorInvalidValue;
+    const bool dec = a.xy.fx * a.xy.fy > 1;
+    if (dec && out_dt != in_dt) return hipErrorInvalidValue;
     auto go = [&](auto tin, auto tout) -> hipError_t {
         using TIn = decltype(tin);
         using TOut = decltype(tout);
@@ -413,7 +422,10 @@ hipError_t launch_cxy_ref(const CXYRefArgs& a, int in_dt, int out_dt, hipStream_
         switch (a.xy.kx) {
 #define NDL_K(KK)                                                                                              \
     case KK:                                                                                                   \
-        hipLaunchKernelGGL((cxy_ref_kernel<TIn, TOut, KK>), dim3(a.xy.ntiles), dim3(256), lds, s, a);          \
+        if (dec)                                                                                               \
+            hipLaunchKernelGGL((cxy_ref_kernel<TIn, TOut, KK, true>), dim3(a.xy.ntiles), dim3(256), lds, s, a);  \
+        else                                                                                                   \
+            hipLaunchKernelGGL((cxy_ref_kernel<TIn, TOut, KK, false>), dim3(a.xy.ntiles), dim3(256), lds, s, a); \
         return hipGetLastError();
             NDL_K(3) NDL_K(5) NDL_K(7) NDL_K(9)
 #undef NDL_K
