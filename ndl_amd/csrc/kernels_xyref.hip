@@ -279,26 +279,35 @@ __global__ __launch_bounds__(256) void cxy_ref_kernel(const CXYRefArgs ca)
         // 1b. c pass of output channel ch over the window; the source channel
         //     of every tap is the same for the whole workgroup
         int soff[K];
-        bool zero[K];
+        bool zero[K], any_zero = false;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int e = border_map32(ca.cb, m, ch + k - ca.cc);
             const bool mid = e >= 1 && e <= 3;
             soff[k] = mid && e < m ? e * (HA * WA) : 0;
             zero[k] = mid && e >= m;
+            any_zero = any_zero || zero[k];
         }
-        for (int idx = threadIdx.x; idx < HA * WA; idx += 256) {
-            TIn raw[K];
+        auto cpass = [&](auto with_zero) {
+            for (int idx = threadIdx.x; idx < HA * WA; idx += 256) {
+                TIn raw[K];
 #pragma unroll
-            for (int k = 0; k < K; k++) raw[k] = win[soff[k] + idx];
-            double total = 0.0;
+                for (int k = 0; k < K; k++) raw[k] = win[soff[k] + idx];
+                double total = 0.0;
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double v = zero[k] ? 0.0 : load_f64(&raw[k]);
-                total = total + v * ca.wc[k];
+                for (int k = 0; k < K; k++) {
+                    double v = load_f64(&raw[k]);
+                    if constexpr (decltype(with_zero)::value) v = zero[k] ? 0.0 : v;
+                    total = total + v * ca.wc[k];
+                }
+                cw[idx] = total;
             }
-            cw[idx] = total;
-        }
+        };
+        // a tap past the axis reads zero only when its reach exceeds the
+        // extent, which the reference's argument check rejects: the common
+        // loop carries no select
+        if (any_zero) cpass(std::true_type{});
+        else cpass(std::false_type{});
         __syncthreads();
 
         // 2. x pass: task (window row r, outputs 4s .. 4s+3)
