@@ -248,26 +248,35 @@ __global__ __launch_bounds__(256) void cxy_ref_kernel(const CXYRefArgs ca)
         const int64_t xo1 = (int64_t)safe_map32(a.bx, a.X, x0 - a.cx + c1) * a.isx;
         const int ybase = y0 - a.cy;
         const bool yin = ybase >= 0 && ybase + HA <= a.Y;
-        for (int ch = 0; ch < m; ch++) {
-            const TIn* in = (const TIn*)a.in + ib + (int64_t)ch * ca.ics;
-            TIn* w = win + ch * (HA * WA);
-            TIn e0[RW], e1[RW];
+        // every channel's loads are issued before the first LDS store
+        TIn e0[kCXYMaxC][RW], e1[kCXYMaxC][RW];
 #pragma unroll
-            for (int j = 0; j < RW; j++) {
-                const int r = wv + 4 * j;
-                if (r < HA) {
-                    const int yy = yin ? ybase + r : safe_map32(a.by, a.Y, ybase + r);
-                    const TIn* row = in + (int64_t)yy * a.isy;
-                    e0[j] = row[xo0];
-                    e1[j] = row[xo1];
+        for (int ch = 0; ch < kCXYMaxC; ch++) {
+            if (ch < m) {
+                const TIn* in = (const TIn*)a.in + ib + (int64_t)ch * ca.ics;
+#pragma unroll
+                for (int j = 0; j < RW; j++) {
+                    const int r = wv + 4 * j;
+                    if (r < HA) {
+                        const int yy = yin ? ybase + r : safe_map32(a.by, a.Y, ybase + r);
+                        const TIn* row = in + (int64_t)yy * a.isy;
+                        e0[ch][j] = row[xo0];
+                        e1[ch][j] = row[xo1];
+                    }
                 }
             }
+        }
 #pragma unroll
-            for (int j = 0; j < RW; j++) {
-                const int r = wv + 4 * j;
-                if (r < HA) {
-                    w[r * WA + lane] = e0[j];
-                    w[r * WA + c1] = e1[j];
+        for (int ch = 0; ch < kCXYMaxC; ch++) {
+            if (ch < m) {
+                TIn* w = win + ch * (HA * WA);
+#pragma unroll
+                for (int j = 0; j < RW; j++) {
+                    const int r = wv + 4 * j;
+                    if (r < HA) {
+                        w[r * WA + lane] = e0[ch][j];
+                        w[r * WA + c1] = e1[ch][j];
+                    }
                 }
             }
         }
